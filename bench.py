@@ -1,0 +1,150 @@
+"""Throughput of the 3D EM-PIC step on one MI355X.
+
+Workload: 128^3 cells x 64 particles/cell/species (268,435,456 particles),
+two species, dipole standoff 32 cells, inflow every 5 steps, cell sort every
+20 steps, synthetic seeded plasma.  A step is one full fdtd() step (B half
+steps, fused gather/push/deposit pass per species, current fold, E step,
+particle boundary, and the amortised inflow and sort when due), with all
+state resident in HBM.
+
+Prints one JSON line: value = particle-steps/s of the chosen form (FP32 by
+default), with the plain FP64 form's rate, the roofline of the dominant
+kernel (the fused particle pass) and a CPU baseline (the C oracle on the
+host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# algorithmic bytes per particle-step of the fused pass (read + write state)
+BYTES_PER_PARTICLE = {"fp64": 96, "fp32": 56}
+
+
+def run_form(ps, form, steps, warmup, nz=None):
+    c = ps.preset(2)
+    c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
+    if nz:
+        c.nz = nz
+    sim = ps.Simulation(c)
+    sim.load_particles()
+    sim.begin()
+    sim.step(warmup)
+    sim.sync()
+    n0 = sim.particle_count(0) + sim.particle_count(1)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    n1 = sim.particle_count(0) + sim.particle_count(1)
+    particles = 0.5 * (n0 + n1)
+    rate = particles * steps / (t1 - t0)
+    # live per-kernel timing of the same step loop (HIP events on the
+    # library's stream), in a separate pass so the timed region above has no
+    # extra event records
+    sim.timing(True)
+    sim.timing_reset()
+    tsteps = max(5, min(steps, 20))
+    sim.step(tsteps)
+    push_ms, push_n = sim.timing_get("push")
+    tot = {k: sim.timing_get(k) for k in ("push", "field_b", "field_e", "sort", "inflow", "other")}
+    sim.timing(False)
+    sim.close()
+    per_launch_particles = particles / 2.0  # one launch per species
+    push_avg_s = push_ms / push_n / 1e3
+    achieved = BYTES_PER_PARTICLE[form] * per_launch_particles / push_avg_s
+    return {
+        "rate": rate, "ms_per_step": (t1 - t0) / steps * 1e3,
+        "particles": particles, "push_avg_ms": push_avg_s * 1e3,
+        "achieved": achieved, "kernel_ms": {k: v[0] / tsteps for k, v in tot.items()},
+    }
+
+
+def cpu_baseline(ps):
+    """The C oracle (FP64, OpenMP) on the host cores: the same physics and
+    particle density on a 128 x 128 x 32 slab of the workload (1/4 of the
+    domain), 3 timed steps after 1 warm-up step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    c = ps.preset(2)
+    c.precision = ps.PIC_FP64
+    c.nz = 32
+    c.absorb[2] = 4
+    c.ballistic_lo[2], c.ballistic_hi[2] = 8.0, 28.0
+    o = Oracle(c, threads=cores)
+    o.load_particles()
+    o.begin()
+    o.step(1)
+    n = o.particle_count(0) + o.particle_count(1)
+    steps = 3
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    o.close()
+    return {"value": n * steps / (t1 - t0), "unit": "particle-steps/s",
+            "cores": cores, "kind": "port",
+            "sample": f"C oracle FP64, 128x128x32 slab x 64 ppc x 2 species "
+                      f"({n} particles), {steps} steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one subdomain per GPU; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+    import plasmasim_amd as ps
+    main_r = run_form(ps, a.form, a.steps, a.warmup)
+    other = None
+    if a.form == "fp32" and not a.no_fp64:
+        other = run_form(ps, "fp64", a.steps, a.warmup)
+    cpu = None if a.no_cpu else cpu_baseline(ps)
+    out = {
+        "metric": "particle-steps/s on 1 MI355X, FP32 vs FP64 form at stated field error",
+        "value": main_r["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": main_r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if a.form == "fp32" else "f64",
+        "data": "synthetic seeded plasma (Philox4x32-10), random-free fields at start",
+        "config": {"workload": "3D EM-PIC step, 128^3 cells x 64 ppc x 2 species "
+                               "(268,435,456 particles), dipole standoff 32, dt 0.45, "
+                               "sort every 20, inflow every 5",
+                   "form": a.form},
+        "roofline": {"bound": "hbm", "achieved": main_r["achieved"] / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": main_r["achieved"] / HBM_PEAK, "traffic": None,
+                     "kernel": "fused particle pass (gather, push, J and n deposit, boundary)",
+                     "kernel_avg_ms": main_r["push_avg_ms"],
+                     "bytes_per_particle": BYTES_PER_PARTICLE[a.form]},
+        "cpu_baseline": cpu,
+        "kernel_ms_per_step": main_r["kernel_ms"],
+    }
+    if other:
+        out["fp64_value"] = other["rate"]
+        out["fp64_ms_per_step"] = other["ms_per_step"]
+        out["fp32_over_fp64"] = main_r["rate"] / other["rate"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

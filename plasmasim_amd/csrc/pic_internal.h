@@ -1,0 +1,227 @@
+// Internal state of a pic_ctx and helpers shared by the HIP translation
+// units.  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/plasmasim.h"
+
+namespace pic {
+
+constexpr int kBlock = 256;  // threads per workgroup for streaming kernels
+
+// Device view of the grid geometry.
+struct Geom {
+  int NX, NY, NZ;  // interior cells
+  int SX, SY, SZ;  // with 2 guards per side
+  int64_t plane;   // elements between consecutive field planes
+  __host__ __device__ inline int64_t idx(int k, int l, int m) const {
+    return ((int64_t)k * SY + l) * SZ + m;
+  }
+};
+
+// Sort tiles: key = tile-major, lexicographic (k,l,m) inside a tile, over
+// interior cell indices.  One tile spanning the interior gives the
+// reference's key ((k*NY)+l)*NZ+m.
+struct TileGeom {
+  int TX, TY, TZ;     // cells per tile
+  int NTX, NTY, NTZ;  // tiles per axis
+  __host__ __device__ inline uint32_t key(int k, int l, int m) const {
+    const uint32_t tile = ((uint32_t)(k / TX) * NTY + l / TY) * NTZ + m / TZ;
+    const uint32_t in = ((uint32_t)(k % TX) * TY + l % TY) * TZ + m % TZ;
+    return tile * (uint32_t)(TX * TY * TZ) + in;
+  }
+  __host__ __device__ inline int ntiles() const { return NTX * NTY * NTZ; }
+};
+
+// Per-axis sigma ingredients on the 2x lattice: depth^4 and depth^12.
+struct SigmaProfile {
+  const double *d4[3];
+  const double *d12[3];
+  double amp;
+};
+
+// Field planes in the order of enum pic_field.  In the FP32 form the B
+// planes hold the induced field B - B0 (the curl only needs B - B0, and
+// storing it avoids the cancellation of B - B0 in single precision near the
+// dipole); downloads add B0 back.
+template <typename T>
+struct Fields {
+  T *f[PIC_NFIELDS];
+};
+
+struct Particles64 {
+  double *x, *y, *z, *vx, *vy, *vz;
+};
+
+// FP32 form: cell index into the guard-inclusive grid, offsets in [0,1).
+struct Particles32 {
+  int32_t *cell;
+  float *ox, *oy, *oz, *vx, *vy, *vz;
+};
+
+// Constants of one step shared by the particle kernels.
+struct StepConst {
+  double dx, dt, c, q;
+  double mass;      // signed: +m_i for ions, -m_e for electrons (:784)
+  double weight;    // p->n
+  double vdrift_z;  // V in cal_track4's IMF terms (:2817-2818)
+  double imf_x, imf_y;
+  double blo[3], bhi[3];  // ballistic zone bounds
+  int periodic[2];
+};
+
+}  // namespace pic
+
+struct pic_ctx {
+  pic_config cfg;
+  pic::Geom geom;
+  hipStream_t stream = nullptr;
+  char err[512];
+
+  // fields: PIC_NFIELDS planes of double (FP64) or float (FP32)
+  void *field_mem = nullptr;
+  size_t field_bytes = 0;
+  double *sig_mem = nullptr;  // 6 x 2*Smax doubles
+  int sig_stride = 0;
+  double *b0_host = nullptr;  // FP32 form: B0 in double, 3 planes
+
+  // particles: two buffers per species, swapped by compaction and sort
+  int64_t cap = 0;
+  int64_t n[2] = {0, 0};  // stored slots, tombstones included
+  void *pbuf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int cur[2] = {0, 0};
+
+  // scratch for compaction and sorting (sized to the particle capacity)
+  uint32_t *keys = nullptr, *keys_alt = nullptr;
+  uint32_t *idx = nullptr, *idx_alt = nullptr;
+  uint8_t *flags = nullptr;
+  void *prim_tmp = nullptr;        // rocPRIM temporary storage
+  size_t prim_tmp_bytes = 0;
+  int64_t *dev_counter = nullptr;  // 8 int64 device scratch slots
+  int64_t *host_counter = nullptr; // pinned mirror
+  double *diag_partial = nullptr;  // reduction partials
+  int64_t diag_partial_cap = 0;
+
+  // step counters (reference fdtd(): c, c_add, c_thrust)
+  int64_t step = 0;
+  int32_t c_add = 0, c_sort = 0;
+  uint32_t inflow_event = 0;
+
+  // timing: pairs of HIP events around launches, resolved in batches
+  static constexpr int kEventPool = 512;
+  int timing = 0;
+  double t_ms[PIC_T_COUNT] = {0};
+  int64_t t_n[PIC_T_COUNT] = {0};
+  hipEvent_t tev[2][kEventPool] = {};
+  int tclass[kEventPool] = {};
+  int tpending = 0;
+};
+
+namespace pic {
+
+int set_error(pic_ctx *ctx, int code, const char *fmt, ...);
+
+#define PIC_HIP(ctx, call)                                                  \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return ::pic::set_error((ctx), PIC_E_HIP, "%s:%d %s: %s", __FILE__,   \
+                              __LINE__, #call, hipGetErrorString(e_));      \
+  } while (0)
+
+#define PIC_LAUNCH_CHECK(ctx) PIC_HIP(ctx, hipGetLastError())
+
+inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+inline Particles64 p64(pic_ctx *ctx, int sp, int buf) {
+  double *b = (double *)ctx->pbuf[sp][buf];
+  const int64_t c = ctx->cap;
+  return Particles64{b, b + c, b + 2 * c, b + 3 * c, b + 4 * c, b + 5 * c};
+}
+inline Particles32 p32(pic_ctx *ctx, int sp, int buf) {
+  char *b = (char *)ctx->pbuf[sp][buf];
+  const int64_t c = ctx->cap;
+  float *f = (float *)(b + 4 * c);
+  return Particles32{(int32_t *)b, f, f + c, f + 2 * c, f + 3 * c, f + 4 * c,
+                     f + 5 * c};
+}
+template <typename T>
+inline Fields<T> fields(pic_ctx *ctx) {
+  Fields<T> F;
+  T *base = (T *)ctx->field_mem;
+  for (int i = 0; i < PIC_NFIELDS; ++i) F.f[i] = base + i * ctx->geom.plane;
+  return F;
+}
+inline SigmaProfile sigma(pic_ctx *ctx) {
+  SigmaProfile s;
+  for (int a = 0; a < 3; ++a) {
+    s.d4[a] = ctx->sig_mem + (2 * a) * ctx->sig_stride;
+    s.d12[a] = ctx->sig_mem + (2 * a + 1) * ctx->sig_stride;
+  }
+  s.amp = ctx->cfg.sigma_amp;
+  return s;
+}
+StepConst step_const(const pic_ctx *ctx, int sp);
+inline TileGeom tile_geom(const pic_ctx *ctx) {
+  const pic_config &c = ctx->cfg;
+  TileGeom t;
+  t.TX = c.tile[0] > 0 ? c.tile[0] : c.nx;
+  t.TY = c.tile[1] > 0 ? c.tile[1] : c.ny;
+  t.TZ = c.tile[2] > 0 ? c.tile[2] : c.nz;
+  t.NTX = c.nx / t.TX;
+  t.NTY = c.ny / t.TY;
+  t.NTZ = c.nz / t.TZ;
+  return t;
+}
+
+// timing helpers (HIP events on the context stream)
+void timing_begin(pic_ctx *ctx);
+void timing_end(pic_ctx *ctx, int which);
+int timing_flush(pic_ctx *ctx);
+
+// per-form implementations (fp64 / fp32 translation units)
+namespace f64 {
+int renew_grid(pic_ctx *);
+int renew_b_half(pic_ctx *);
+int halo(pic_ctx *, int first);
+int fold_current(pic_ctx *);
+int renew_e(pic_ctx *);
+int load_particles(pic_ctx *);
+int push(pic_ctx *, int sp, int do_push_j, int do_drift_n, int do_boundary);
+int particle_boundary(pic_ctx *, int sp);
+int inflow(pic_ctx *, int sp, uint32_t event);
+int sort(pic_ctx *, int sp);
+int diagnose(pic_ctx *, pic_diag *);
+int count_live(pic_ctx *, int sp, int64_t *n);
+}  // namespace f64
+namespace f32 {
+int renew_grid(pic_ctx *);
+int renew_b_half(pic_ctx *);
+int halo(pic_ctx *, int first);
+int fold_current(pic_ctx *);
+int renew_e(pic_ctx *);
+int load_particles(pic_ctx *);
+int push(pic_ctx *, int sp, int do_push_j, int do_drift_n, int do_boundary);
+int particle_boundary(pic_ctx *, int sp);
+int inflow(pic_ctx *, int sp, uint32_t event);
+int sort(pic_ctx *, int sp);
+int diagnose(pic_ctx *, pic_diag *);
+int count_live(pic_ctx *, int sp, int64_t *n);
+int upload_particles(pic_ctx *, int sp, int64_t n, const double *const *h);
+int download_particles(pic_ctx *, int sp, int64_t cap, int64_t *n,
+                       double *const *h);
+int finish_deposit(pic_ctx *);
+}  // namespace f32
+
+// device-wide building blocks on rocPRIM (pic_prim.hip)
+// stable selection: idx_out[0..*count) = indices i < n with flags[i] != 0
+int select_flagged(pic_ctx *ctx, const uint8_t *flags, uint32_t *idx_out,
+                   int64_t n, int64_t *count);
+// stable key sort of (key, index) pairs; indices start as 0..n-1
+int sort_keys(pic_ctx *ctx, uint32_t *keys_in, uint32_t *keys_out,
+              uint32_t *idx_out, int64_t n, int end_bit);
+int ensure_prim_tmp(pic_ctx *ctx, size_t bytes);
+
+}  // namespace pic

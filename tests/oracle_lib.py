@@ -1,0 +1,163 @@
+"""ctypes wrapper of the CPU oracle (oracle/libpic_oracle.so).
+
+Test infrastructure: exposes the same method names as
+plasmasim_amd.Simulation so parity tests drive both through one routine
+sequence.  The oracle is built by `make -C oracle` when the library is
+missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from plasmasim_amd._lib import FIELD_ID, PicConfig, PicDiag, dptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "libpic_oracle.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_P = ctypes.c_void_p
+_SIG = {
+    "orc_create": (_P, [ctypes.POINTER(PicConfig)]),
+    "orc_destroy": (None, [_P]),
+    "orc_threads": (ctypes.c_int, [ctypes.c_int]),
+    "orc_upload_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "orc_download_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "orc_upload_particles": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64] + [_D] * 6),
+    "orc_download_particles": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64,
+                                              ctypes.POINTER(ctypes.c_int64)] + [_D] * 6),
+    "orc_particle_count": (ctypes.c_int64, [_P, ctypes.c_int]),
+    "orc_load_particles": (ctypes.c_int, [_P]),
+    "orc_renew_grid": (ctypes.c_int, [_P]),
+    "orc_renew_b_half": (ctypes.c_int, [_P]),
+    "orc_halo_b": (ctypes.c_int, [_P]),
+    "orc_halo_e": (ctypes.c_int, [_P]),
+    "orc_push_deposit_j": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc_drift_deposit_n": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc_fold_current": (ctypes.c_int, [_P]),
+    "orc_renew_e": (ctypes.c_int, [_P]),
+    "orc_particle_boundary": (ctypes.c_int, [_P]),
+    "orc_inflow": (ctypes.c_int, [_P]),
+    "orc_sort": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc_diagnose": (ctypes.c_int, [_P, ctypes.POINTER(PicDiag)]),
+    "orc_begin": (ctypes.c_int, [_P]),
+    "orc_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_uint32)]),
+    "orc_gather": (None, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
+    "orc_boris": (None, [ctypes.POINTER(PicConfig), ctypes.c_double, _D, _D]),
+    "orc_sigma": (ctypes.c_double, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+}
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        src = os.path.join(ORACLE_DIR, "pic_oracle.c")
+        if (not os.path.exists(ORACLE_SO)
+                or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src)):
+            subprocess.run(["make", "-C", ORACLE_DIR], check=True,
+                           capture_output=True)
+        _LIB = ctypes.CDLL(ORACLE_SO)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(_LIB, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _LIB
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def boris(cfg, m, fields, p):
+    f = np.ascontiguousarray(fields, dtype=np.float64)
+    s = np.ascontiguousarray(p, dtype=np.float64).copy()
+    lib().orc_boris(ctypes.byref(cfg), m, dptr(f), dptr(s))
+    return s
+
+
+class Oracle:
+    """Same surface as plasmasim_amd.Simulation, backed by the CPU oracle."""
+
+    def __init__(self, cfg, threads=1):
+        self._L = lib()
+        self._L.orc_threads(threads)
+        self.cfg = cfg.copy()
+        self._o = self._L.orc_create(ctypes.byref(self.cfg))
+        if not self._o:
+            raise MemoryError("orc_create failed")
+        self.shape = (cfg.nx + 4, cfg.ny + 4, cfg.nz + 4)
+
+    def close(self):
+        if self._o:
+            self._L.orc_destroy(self._o)
+            self._o = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"oracle error {rc}")
+
+    def upload_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape
+        self._check(self._L.orc_upload_field(self._o, FIELD_ID[name], dptr(a)))
+
+    def download_field(self, name):
+        a = np.empty(self.shape)
+        self._check(self._L.orc_download_field(self._o, FIELD_ID[name], dptr(a)))
+        return a
+
+    def upload_particles(self, sp, parts):
+        p = np.asarray(parts, dtype=np.float64)
+        cols = [np.ascontiguousarray(p[:, j]) for j in range(6)]
+        self._check(self._L.orc_upload_particles(self._o, sp, len(p),
+                                                 *[dptr(c) for c in cols]))
+
+    def download_particles(self, sp):
+        n = self._L.orc_particle_count(self._o, sp)
+        cols = [np.empty(max(n, 1)) for _ in range(6)]
+        nn = ctypes.c_int64()
+        self._check(self._L.orc_download_particles(
+            self._o, sp, max(n, 1), ctypes.byref(nn), *[dptr(c) for c in cols]))
+        return np.stack([c[:nn.value] for c in cols], axis=1)
+
+    def particle_count(self, sp):
+        return self._L.orc_particle_count(self._o, sp)
+
+    def gather(self, x, y, z):
+        out = np.empty(6)
+        self._L.orc_gather(self._o, x, y, z, dptr(out))
+        return out
+
+    def sigma(self, k2, l2, m2):
+        return self._L.orc_sigma(self._o, k2, l2, m2)
+
+    def diagnose(self):
+        d = PicDiag()
+        self._check(self._L.orc_diagnose(self._o, ctypes.byref(d)))
+        return d.to_dict()
+
+    def __getattr__(self, name):
+        # routine methods: load_particles, renew_grid, ..., step(n), sort(sp)
+        fn = getattr(self._L, "orc_" + name, None)
+        if fn is None:
+            raise AttributeError(name)
+
+        def call(*args):
+            self._check(fn(self._o, *args))
+        return call
