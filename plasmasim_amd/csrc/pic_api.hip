@@ -189,6 +189,11 @@ static int validate(const pic_config *c, char *err, size_t n) {
     snprintf(err, n, "unknown deposit mode");
     return PIC_E_ARG;
   }
+  if (c->precision == PIC_FP32 &&
+      (c->nx + 4 > 1023 || c->ny + 4 > 1023 || c->nz + 4 > 1023)) {
+    snprintf(err, n, "FP32 form packs cells in 10 bits per axis: n <= 1019");
+    return PIC_E_ARG;
+  }
   if (c->inflow_every < 1 || c->sort_every < 1) {
     snprintf(err, n, "cadences must be >= 1");
     return PIC_E_ARG;
@@ -296,6 +301,7 @@ static void free_all(pic_ctx *ctx) {
       if (ctx->tev[j][i]) (void)hipEventDestroy(ctx->tev[j][i]);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   free(ctx->b0_host);
+  if (ctx->cfg.precision == PIC_FP32) f32::free_state(ctx);
 }
 
 // ---------------------------------------------------------------------------
@@ -429,6 +435,15 @@ extern "C" int pic_create(const pic_config *cfg, pic_ctx **out) {
     for (int j = 0; j < 2; ++j) CREATE_HIP(hipEventCreate(&ctx->tev[j][i]));
   CREATE_HIP(hipStreamSynchronize(ctx->stream));
 #undef CREATE_HIP
+  if (!f64) {
+    int rc2 = f32::init_state(ctx);
+    if (rc2) {
+      fprintf(stderr, "pic_create: %s\n", ctx->err);
+      free_all(ctx);
+      delete ctx;
+      return rc2;
+    }
+  }
   *out = ctx;
   return 0;
 }

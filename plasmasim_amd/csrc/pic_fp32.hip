@@ -1,33 +1,1557 @@
-// FP32 chosen form (placeholder until implemented): every entry reports that
-// the form is unavailable instead of computing anything.
+// FP32 chosen form of the step on gfx950.
+//
+// Particles: packed cell (k<<20 | l<<10 | m, guard-inclusive indices, -1 for
+// a dead slot), fp32 in-cell offsets in [0,1) and fp32 velocities: 28 bytes,
+// positions uniformly resolved to 2^-24 cell anywhere in the box.  Fields:
+// fp32 planes; the B planes hold the induced field B - B0 (B0 is static), so
+// the curl never forms B - B0 in single precision next to the dipole.
+//
+// The particle pass runs one workgroup per sort tile.  Particles of a tile
+// are contiguous after the sort; the workgroup accumulates their current and
+// density in an LDS window (tile +- 3 cells) of 64-bit fixed-point integers
+// (on gfx950 integer LDS atomics are an order of magnitude faster than fp32
+// ones, and their sums do not depend on the order of the additions), then
+// adds the non-zero window nodes to the global planes.  A particle that
+// drifted more than one cell out of its tile since the sort, or a particle of
+// the inflow tail outside its chunk's window, deposits with global atomics
+// directly.  Atomic mode: global fp32 atomics into the fp32 planes.
+// Deterministic mode: global 64-bit integer atomics into fixed-point planes,
+// converted to fp32 after the pass, so results are bit-reproducible.
 #include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
 
+#include <algorithm>
+
+#include "philox.h"
 #include "pic_internal.h"
 
 namespace pic {
 namespace f32 {
-#define NA(ctx) set_error((ctx), PIC_E_STATE, "FP32 form not built")
-int renew_grid(pic_ctx *c) { return NA(c); }
-int renew_b_half(pic_ctx *c) { return NA(c); }
-int halo(pic_ctx *c, int) { return NA(c); }
-int fold_current(pic_ctx *c) { return NA(c); }
-int renew_e(pic_ctx *c) { return NA(c); }
-int load_particles(pic_ctx *c) { return NA(c); }
-int push(pic_ctx *c, int, int, int, int) { return NA(c); }
-int particle_boundary(pic_ctx *c, int) { return NA(c); }
-int inflow(pic_ctx *c, int, uint32_t) { return NA(c); }
-int sort(pic_ctx *c, int) { return NA(c); }
-int diagnose(pic_ctx *c, pic_diag *) { return NA(c); }
-int count_live(pic_ctx *c, int, int64_t *) { return NA(c); }
-int upload_particles(pic_ctx *c, int, int64_t, const double *const *) { return NA(c); }
-int download_particles(pic_ctx *c, int, int64_t, int64_t *, double *const *) { return NA(c); }
-int finish_deposit(pic_ctx *) { return 0; }
+
+constexpr int kMargin = 3;      // window margin in cells (1 drift + 2 stencil)
+constexpr int kWin = 1536;      // LDS window capacity in nodes (4 x int64 each)
+constexpr int kTailChunk = 2048;
+constexpr int kPackShift = 10;
+constexpr int kPackMask = 1023;
+
+__device__ __host__ inline int32_t pack(int k, int l, int m) {
+  return (k << 20) | (l << 10) | m;
+}
+__device__ inline void unpack(int32_t c, int &k, int &l, int &m) {
+  k = c >> 20;
+  l = (c >> kPackShift) & kPackMask;
+  m = c & kPackMask;
+}
+
+// constants of the particle pass for one species, fp32
+struct Push32 {
+  float T;          // q dt / (2 m), m signed
+  float hdt;        // dt / (2 dx): offset change per unit velocity per half step
+  float c, c2;
+  float qw;         // q w / dx^3: current per unit velocity
+  float w;          // w / dx^3: density
+  float eimf_x, eimf_y;  // V * IMF terms added to the gathered E (:2817-2818)
+  float blo[3], bhi[3];  // ballistic zone bounds in cell units
+  int periodic[2];
+  double fx_scale;  // deterministic mode: fixed-point units per unit value
+};
+
+// grid of planes used by the pass
+struct Grid32 {
+  Geom g;
+  const float *ex, *ey, *ez, *bx, *by, *bz, *b0x, *b0y, *b0z;
+};
+
+struct TileRun {
+  const uint32_t *tile_start;  // ntiles + 1 entries
+  int ntiles;
+  int64_t sorted_end;          // [0, sorted_end) is tile-sorted
+  int64_t n;                   // stored slots
+  int ntail;                   // tail chunks of kTailChunk
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+
+__device__ inline int xcd_remap(int b, int nb) {
+  // consecutive workgroups are dealt round-robin over the 8 XCDs; give each
+  // XCD a contiguous range of tiles so neighbouring tiles share its L2
+  const int q = nb / 8;
+  if (b >= 8 * q) return b;
+  return (b % 8) * q + b / 8;
+}
+
+struct AxisW32 {
+  int bi, bh;
+  float i0, i1, h0, h1;
+};
+
+__device__ inline AxisW32 axis_w(int k, float o) {
+  AxisW32 a;
+  a.bi = k;
+  a.i1 = o;
+  a.i0 = 1.0f - o;
+  if (o >= 0.5f) {
+    a.bh = k;
+    a.h1 = o - 0.5f;
+  } else {
+    a.bh = k - 1;
+    a.h1 = o + 0.5f;
+  }
+  a.h0 = 1.0f - a.h1;
+  return a;
+}
+
+__device__ inline float lerp8(const float *__restrict__ F, int64_t i, int64_t dl,
+                              int64_t dk, float a0, float a1, float b0, float b1,
+                              float c0, float c1) {
+  const float v00 = c0 * F[i] + c1 * F[i + 1];
+  const float v01 = c0 * F[i + dl] + c1 * F[i + dl + 1];
+  const float v10 = c0 * F[i + dk] + c1 * F[i + dk + 1];
+  const float v11 = c0 * F[i + dk + dl] + c1 * F[i + dk + dl + 1];
+  return a0 * (b0 * v00 + b1 * v01) + a1 * (b0 * v10 + b1 * v11);
+}
+
+__device__ inline float lerp8s(const float *__restrict__ F,
+                               const float *__restrict__ G, int64_t i,
+                               int64_t dl, int64_t dk, float a0, float a1,
+                               float b0, float b1, float c0, float c1) {
+#define S(j) (F[i + (j)] + G[i + (j)])
+  const float v00 = c0 * S(0) + c1 * S(1);
+  const float v01 = c0 * S(dl) + c1 * S(dl + 1);
+  const float v10 = c0 * S(dk) + c1 * S(dk + 1);
+  const float v11 = c0 * S(dk + dl) + c1 * S(dk + dl + 1);
+#undef S
+  return a0 * (b0 * v00 + b1 * v01) + a1 * (b0 * v10 + b1 * v11);
+}
+
+// trilinear gather on the six staggered lattices (shape_func_v2
+// :3017-3326); B is B0 + B1
+__device__ inline void gather32(const Grid32 &G, const AxisW32 &X,
+                                const AxisW32 &Y, const AxisW32 &Z, float f[6]) {
+  const Geom &g = G.g;
+  const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+  f[0] = lerp8(G.ex, g.idx(X.bi, Y.bh, Z.bh), dl, dk, X.i0, X.i1, Y.h0, Y.h1, Z.h0, Z.h1);
+  f[1] = lerp8(G.ey, g.idx(X.bh, Y.bi, Z.bh), dl, dk, X.h0, X.h1, Y.i0, Y.i1, Z.h0, Z.h1);
+  f[2] = lerp8(G.ez, g.idx(X.bh, Y.bh, Z.bi), dl, dk, X.h0, X.h1, Y.h0, Y.h1, Z.i0, Z.i1);
+  f[3] = lerp8s(G.bx, G.b0x, g.idx(X.bh, Y.bi, Z.bi), dl, dk, X.h0, X.h1, Y.i0, Y.i1, Z.i0, Z.i1);
+  f[4] = lerp8s(G.by, G.b0y, g.idx(X.bi, Y.bh, Z.bi), dl, dk, X.i0, X.i1, Y.h0, Y.h1, Z.i0, Z.i1);
+  f[5] = lerp8s(G.bz, G.b0z, g.idx(X.bi, Y.bi, Z.bh), dl, dk, X.i0, X.i1, Y.i0, Y.i1, Z.h0, Z.h1);
+}
+
+// cal_track4 (:2804-2870) in fp32: B scaled by 1/gamma(u^{n-1/2}), gamma1
+// set to 0 when |v| >= c
+__device__ inline void boris32(const Push32 &p, const float f[6], float &vx,
+                               float &vy, float &vz) {
+  const float T = p.T;
+  const float ex = f[0] + p.eimf_x, ey = f[1] + p.eimf_y, ez = f[2];
+  const float d = p.c2 - (vx * vx + vy * vy + vz * vz);
+  const float g1 = d > 0.0f ? p.c * rsqrtf(d) : 0.0f;
+  float ux = g1 * vx, uy = g1 * vy, uz = g1 * vz;
+  const float g2 = p.c * rsqrtf(p.c2 + ux * ux + uy * uy + uz * uz);
+  const float bx = g2 * f[3], by = g2 * f[4], bz = g2 * f[5];
+  const float S = __fdividef(2.0f * T, 1.0f + T * T * (bx * bx + by * by + bz * bz));
+  ux += ex * T;
+  uy += ey * T;
+  uz += ez * T;
+  const float ox = ux + T * (uy * bz - uz * by);
+  const float oy = uy + T * (uz * bx - ux * bz);
+  const float oz = uz + T * (ux * by - uy * bx);
+  ux += S * (oy * bz - oz * by) + ex * T;
+  uy += S * (oz * bx - ox * bz) + ey * T;
+  uz += S * (ox * by - oy * bx) + ez * T;
+  const float g = p.c * rsqrtf(p.c2 + ux * ux + uy * uy + uz * uz);
+  vx = g * ux;
+  vy = g * uy;
+  vz = g * uz;
+}
+
+// half-step drift of one axis: offset += v dt/(2dx), at most one cell crossed
+__device__ inline void drift_axis(int &k, float &o, float v, float hdt) {
+  o += v * hdt;
+  if (o >= 1.0f) {
+    o -= 1.0f;
+    k += 1;
+  } else if (o < 0.0f) {
+    o += 1.0f;
+    k -= 1;
+    if (o >= 1.0f) o = 0.99999994f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// accumulation targets
+
+// Atomics with explicit address spaces: the LDS adds must stay ds_add_* and
+// the global ones global_atomic_*; through generic pointers the compiler may
+// merge both paths into flat atomics.
+#define PIC_LDS __attribute__((address_space(3)))
+#define PIC_GLB __attribute__((address_space(1)))
+
+typedef unsigned long long u64;
+
+// fixed-point value of a contribution
+__device__ inline u64 to_fx(float v, double s) {
+  return (u64)__double2ll_rn((double)v * s);
+}
+
+__device__ inline void lds_add(u64 *p, float v, double s) {
+  __hip_atomic_fetch_add((PIC_LDS u64 *)p, to_fx(v, s), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// global target: fp32 planes (atomic mode) or fixed-point planes
+template <bool DET>
+struct GlobT;
+template <>
+struct GlobT<false> {
+  using T = float;
+};
+template <>
+struct GlobT<true> {
+  using T = u64;
+};
+
+__device__ inline void glob_add(float *p, float v, double) {
+  __hip_atomic_fetch_add((PIC_GLB float *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void glob_add(u64 *p, float v, double s) {
+  __hip_atomic_fetch_add((PIC_GLB u64 *)p, to_fx(v, s), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+// flush of one LDS node
+__device__ inline void glob_flush(float *p, u64 v, double inv_s) {
+  __hip_atomic_fetch_add((PIC_GLB float *)p, (float)((double)(long long)v * inv_s),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void glob_flush(u64 *p, u64 v, double) {
+  __hip_atomic_fetch_add((PIC_GLB u64 *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// window in LDS: origin (cells) and extents (nodes)
+struct Win {
+  int k0, l0, m0, WX, WY, WZ;
+  int vk0, vk1, vl0, vl1, vm0, vm1;  // cells whose stencils fit the window
+  bool on;
+};
+
+// deposit of the 8 nodes of one trilinear stencil
+template <typename G>
+__device__ inline void dep8(u64 *lds, G *glob, bool in_lds, const Win &w,
+                            const Geom &g, int a, int b, int c, float a0,
+                            float a1, float b0, float b1, float c0, float c1,
+                            float q, double s) {
+  const float qa0 = q * a0, qa1 = q * a1;
+  const float v[8] = {qa0 * (b0 * c0), qa0 * (b0 * c1), qa0 * (b1 * c0),
+                      qa0 * (b1 * c1), qa1 * (b0 * c0), qa1 * (b0 * c1),
+                      qa1 * (b1 * c0), qa1 * (b1 * c1)};
+  if (in_lds) {
+    const int WZ = w.WZ, WYZ = w.WY * w.WZ;
+    const int i = ((a - w.k0) * w.WY + (b - w.l0)) * WZ + (c - w.m0);
+    lds_add(lds + i, v[0], s);
+    lds_add(lds + i + 1, v[1], s);
+    lds_add(lds + i + WZ, v[2], s);
+    lds_add(lds + i + WZ + 1, v[3], s);
+    lds_add(lds + i + WYZ, v[4], s);
+    lds_add(lds + i + WYZ + 1, v[5], s);
+    lds_add(lds + i + WYZ + WZ, v[6], s);
+    lds_add(lds + i + WYZ + WZ + 1, v[7], s);
+  } else {
+    const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+    const int64_t i = g.idx(a, b, c);
+    glob_add(glob + i, v[0], s);
+    glob_add(glob + i + 1, v[1], s);
+    glob_add(glob + i + dl, v[2], s);
+    glob_add(glob + i + dl + 1, v[3], s);
+    glob_add(glob + i + dk, v[4], s);
+    glob_add(glob + i + dk + 1, v[5], s);
+    glob_add(glob + i + dk + dl, v[6], s);
+    glob_add(glob + i + dk + dl + 1, v[7], s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the fused particle pass
+
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET>
+__global__ __launch_bounds__(256) void k_particles32(
+    Grid32 G, TileGeom tg, Particles32 P, TileRun tr, Push32 kp,
+    typename GlobT<DET>::T *jx, typename GlobT<DET>::T *jy,
+    typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd) {
+  using A = typename GlobT<DET>::T;
+  __shared__ u64 acc[4 * kWin];
+  __shared__ Win sw;
+  __shared__ int64_t s_range[2];
+  const Geom &g = G.g;
+  const int tid = threadIdx.x;
+  const int nb = gridDim.x;
+  const int bid = xcd_remap(blockIdx.x, nb);
+
+  if (tid == 0) {
+    int64_t s, e;
+    Win w;
+    int ak0, ak1, al0, al1, am0, am1;  // anchor box (cells)
+    if (bid < tr.ntiles) {
+      s = tr.tile_start[bid];
+      e = tr.tile_start[bid + 1];
+      const int tz = bid % tg.NTZ, ty = (bid / tg.NTZ) % tg.NTY,
+                tx = bid / (tg.NTZ * tg.NTY);
+      ak0 = 2 + tx * tg.TX;
+      ak1 = ak0 + tg.TX - 1;
+      al0 = 2 + ty * tg.TY;
+      al1 = al0 + tg.TY - 1;
+      am0 = 2 + tz * tg.TZ;
+      am1 = am0 + tg.TZ - 1;
+    } else {
+      const int c = bid - tr.ntiles;
+      s = tr.sorted_end + (int64_t)c * kTailChunk;
+      e = s + kTailChunk < tr.n ? s + kTailChunk : tr.n;
+      // anchor: bounding box of 8 sample particles of the chunk
+      ak0 = al0 = am0 = 1 << 20;
+      ak1 = al1 = am1 = -1;
+      for (int j = 0; j < 8; ++j) {
+        const int64_t i = s + (e - s - 1) * j / 7;
+        if (i < s || i >= e) continue;
+        const int32_t cl = P.cell[i];
+        if (cl < 0) continue;
+        int k, l, m;
+        unpack(cl, k, l, m);
+        ak0 = min(ak0, k); ak1 = max(ak1, k);
+        al0 = min(al0, l); al1 = max(al1, l);
+        am0 = min(am0, m); am1 = max(am1, m);
+      }
+    }
+    w.on = ak1 >= ak0;
+    if (w.on) {
+      w.k0 = ak0 - kMargin;
+      w.l0 = al0 - kMargin;
+      w.m0 = am0 - kMargin;
+      w.WX = ak1 - ak0 + 1 + 2 * kMargin;
+      w.WY = al1 - al0 + 1 + 2 * kMargin;
+      w.WZ = am1 - am0 + 1 + 2 * kMargin;
+      w.vk0 = ak0 - (kMargin - 2);
+      w.vk1 = ak1 + (kMargin - 2);
+      w.vl0 = al0 - (kMargin - 2);
+      w.vl1 = al1 + (kMargin - 2);
+      w.vm0 = am0 - (kMargin - 2);
+      w.vm1 = am1 + (kMargin - 2);
+      if ((int64_t)w.WX * w.WY * w.WZ > kWin) w.on = false;
+    }
+    if (!(PUSH_J || DRIFT_N)) w.on = false;
+    sw = w;
+    s_range[0] = s;
+    s_range[1] = e;
+  }
+  __syncthreads();
+  const int64_t s = s_range[0], e = s_range[1];
+  if (s >= e) return;
+  const Win w = sw;
+  const int WN = w.on ? w.WX * w.WY * w.WZ : 0;
+  for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin + i % WN] = 0;
+  __syncthreads();
+
+  for (int64_t base = s; base < e; base += 256) {
+    const int64_t i = base + tid;
+    if (i >= e) continue;
+    const int32_t cl = P.cell[i];
+    if (cl < 0) continue;
+    int k, l, m;
+    unpack(cl, k, l, m);
+    float ox = P.ox[i], oy = P.oy[i], oz = P.oz[i];
+    float vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+    const bool lds_ok = w.on && k >= w.vk0 && k <= w.vk1 && l >= w.vl0 &&
+                        l <= w.vl1 && m >= w.vm0 && m <= w.vm1;
+    if (PUSH_J) {
+      const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
+      const bool safe = k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 &&
+                        m >= 1 && m <= g.SZ - 2;
+      if (safe && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
+                    yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2])) {
+        float f[6];
+        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+        gather32(G, X, Y, Z, f);
+        boris32(kp, f, vx, vy, vz);
+      }
+      drift_axis(k, ox, vx, kp.hdt);
+      drift_axis(l, oy, vy, kp.hdt);
+      drift_axis(m, oz, vz, kp.hdt);
+      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
+          m <= g.SZ - 2) {
+        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+        dep8(acc + 0 * kWin, jx, lds_ok, w, g, X.bi, Y.bh, Z.bh, X.i0,
+                  X.i1, Y.h0, Y.h1, Z.h0, Z.h1, kp.qw * vx, kp.fx_scale);
+        dep8(acc + 1 * kWin, jy, lds_ok, w, g, X.bh, Y.bi, Z.bh, X.h0,
+                  X.h1, Y.i0, Y.i1, Z.h0, Z.h1, kp.qw * vy, kp.fx_scale);
+        dep8(acc + 2 * kWin, jz, lds_ok, w, g, X.bh, Y.bh, Z.bi, X.h0,
+                  X.h1, Y.h0, Y.h1, Z.i0, Z.i1, kp.qw * vz, kp.fx_scale);
+      }
+    }
+    if (DRIFT_N) {
+      drift_axis(k, ox, vx, kp.hdt);
+      drift_axis(l, oy, vy, kp.hdt);
+      drift_axis(m, oz, vz, kp.hdt);
+      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
+          m <= g.SZ - 2) {
+        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+        dep8(acc + 3 * kWin, nd, lds_ok, w, g, X.bh, Y.bh, Z.bh, X.h0,
+                  X.h1, Y.h0, Y.h1, Z.h0, Z.h1, kp.w, kp.fx_scale);
+      }
+    }
+    int32_t out = pack(k, l, m);
+    if (BOUNDARY) {
+      // sr_particle (:2041-2531): z must stay in [2, NZ+2) cells; x and y
+      // wrap periodically (shift by the box length) or drop
+      bool dead = !(m >= 2 && m < g.NZ + 2) || k < 0 || l < 0 || k >= g.SX ||
+                  l >= g.SY;
+      if (!dead) {
+        if (k >= g.NX + 2) {
+          if (kp.periodic[0]) k -= g.NX; else dead = true;
+        } else if (k < 2) {
+          if (kp.periodic[0]) k += g.NX; else dead = true;
+        }
+        if (l >= g.NY + 2) {
+          if (kp.periodic[1]) l -= g.NY; else dead = true;
+        } else if (l < 2) {
+          if (kp.periodic[1]) l += g.NY; else dead = true;
+        }
+      }
+      out = dead ? -1 : pack(k, l, m);
+    } else if (k < 0 || l < 0 || m < 0 || k >= g.SX || l >= g.SY || m >= g.SZ) {
+      out = -1;  // left the array: not representable
+    }
+    P.cell[i] = out;
+    if (out >= 0) {
+      P.ox[i] = ox;
+      P.oy[i] = oy;
+      P.oz[i] = oz;
+      P.vx[i] = vx;
+      P.vy[i] = vy;
+      P.vz[i] = vz;
+    }
+  }
+  if (!w.on) return;
+  __syncthreads();
+  // flush the window: non-zero nodes inside the array go to the planes
+  const int WYZ = w.WY * w.WZ;
+  const double inv_s = 1.0 / kp.fx_scale;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    A *dst = c == 0 ? jx : c == 1 ? jy : c == 2 ? jz : nd;
+    for (int j = tid; j < WN; j += blockDim.x) {
+      const u64 v = acc[c * kWin + j];
+      if (v == 0) continue;
+      const int kk = w.k0 + j / WYZ, ll = w.l0 + (j / w.WZ) % w.WY,
+                mm = w.m0 + j % w.WZ;
+      if (kk < 0 || ll < 0 || mm < 0 || kk >= g.SX || ll >= g.SY || mm >= g.SZ)
+        continue;
+      glob_flush(dst + g.idx(kk, ll, mm), v, inv_s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// field kernels (fp32; B planes hold B1 = B - B0)
+
+__global__ void k_zero32(float *__restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0.0f;
+}
+
+__global__ void k_zero64(unsigned long long *__restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0ull;
+}
+
+struct Sigma32 {
+  const float *d4[3], *d12[3];
+  float amp;
+};
+
+__device__ inline float sigma32(const Sigma32 &s, int k2, int l2, int m2) {
+  const float a4 = s.d4[0][k2] + s.d4[1][l2] + s.d4[2][m2];
+  if (a4 == 0.0f) return 0.0f;
+  const float a12 = s.d12[0][k2] + s.d12[1][l2] + s.d12[2][m2];
+  return s.amp * (0.1f * sqrtf(a4) + 0.01f * sqrtf(a12));
+}
+
+// renew_b4_2 (:2580-2611) on B1: B1 <- f B1 - h (27 D1 - D3)/24/dx
+__global__ __launch_bounds__(256) void k_renew_b32(Geom g, Fields<float> F,
+                                                   Sigma32 sg, float dt,
+                                                   float dx, float eps0) {
+  const int m = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (m >= g.NZ + 2 || l >= g.NY + 2) return;
+  const float temp = 0.25f * dt / eps0, dth = 0.5f * dt;
+  const float c24 = 1.0f / (24.0f * dx);
+  const int64_t c = g.idx(k, l, m);
+  const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+  const float *ex = F.f[PIC_EX], *ey = F.f[PIC_EY], *ez = F.f[PIC_EZ];
+  float s, f, h;
+  s = sigma32(sg, 2 * k + 1, 2 * l, 2 * m);
+  f = s == 0.0f ? 1.0f : (1.f - s * temp) / (1.f + s * temp);
+  h = s == 0.0f ? dth : dth / (1.f + s * temp);
+  F.f[PIC_BX][c] = f * F.f[PIC_BX][c] -
+                   h * c24 * (27.f * (ez[c] - ez[c - dl] - ey[c] + ey[c - 1]) -
+                              (ez[c + dl] - ez[c - 2 * dl] - ey[c + 1] + ey[c - 2]));
+  s = sigma32(sg, 2 * k, 2 * l + 1, 2 * m);
+  f = s == 0.0f ? 1.0f : (1.f - s * temp) / (1.f + s * temp);
+  h = s == 0.0f ? dth : dth / (1.f + s * temp);
+  F.f[PIC_BY][c] = f * F.f[PIC_BY][c] -
+                   h * c24 * (27.f * (ex[c] - ex[c - 1] - ez[c] + ez[c - dk]) -
+                              (ex[c + 1] - ex[c - 2] - ez[c + dk] + ez[c - 2 * dk]));
+  s = sigma32(sg, 2 * k, 2 * l, 2 * m + 1);
+  f = s == 0.0f ? 1.0f : (1.f - s * temp) / (1.f + s * temp);
+  h = s == 0.0f ? dth : dth / (1.f + s * temp);
+  F.f[PIC_BZ][c] = f * F.f[PIC_BZ][c] -
+                   h * c24 * (27.f * (ey[c] - ey[c - dk] - ex[c] + ex[c - dl]) -
+                              (ey[c + dk] - ey[c - 2 * dk] - ex[c + dl] + ex[c - 2 * dl]));
+}
+
+// renew_e4_2 (:2613-2652) with the curl of B1
+__global__ __launch_bounds__(256) void k_renew_e32(Geom g, Fields<float> F,
+                                                   Sigma32 sg, float dt,
+                                                   float dx, float c_light,
+                                                   float mu0) {
+  const int m = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (m >= g.NZ + 2 || l >= g.NY + 2) return;
+  const float temp = c_light * c_light * dt;
+  const float c24 = 1.0f / (24.0f * dx);
+  const int64_t c = g.idx(k, l, m);
+  const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+  const float *bx = F.f[PIC_BX], *by = F.f[PIC_BY], *bz = F.f[PIC_BZ];
+  float s, f, h;
+  s = sigma32(sg, 2 * k, 2 * l + 1, 2 * m + 1);
+  f = s == 0.0f ? 1.0f : (1.f - 0.5f * mu0 * s * temp) / (1.f + 0.5f * mu0 * s * temp);
+  h = s == 0.0f ? temp : temp / (1.f + 0.5f * mu0 * s * temp);
+  F.f[PIC_EX][c] = f * F.f[PIC_EX][c] +
+                   h * (c24 * (27.f * ((bz[c + dl] - bz[c]) - (by[c + 1] - by[c])) -
+                               ((bz[c + 2 * dl] - bz[c - dl]) - (by[c + 2] - by[c - 1]))) -
+                        mu0 * (F.f[PIC_JIX][c] - F.f[PIC_JEX][c]));
+  s = sigma32(sg, 2 * k + 1, 2 * l, 2 * m + 1);
+  f = s == 0.0f ? 1.0f : (1.f - 0.5f * mu0 * s * temp) / (1.f + 0.5f * mu0 * s * temp);
+  h = s == 0.0f ? temp : temp / (1.f + 0.5f * mu0 * s * temp);
+  F.f[PIC_EY][c] = f * F.f[PIC_EY][c] +
+                   h * (c24 * (27.f * ((bx[c + 1] - bx[c]) - (bz[c + dk] - bz[c])) -
+                               ((bx[c + 2] - bx[c - 1]) - (bz[c + 2 * dk] - bz[c - dk]))) -
+                        mu0 * (F.f[PIC_JIY][c] - F.f[PIC_JEY][c]));
+  s = sigma32(sg, 2 * k + 1, 2 * l + 1, 2 * m);
+  f = s == 0.0f ? 1.0f : (1.f - 0.5f * mu0 * s * temp) / (1.f + 0.5f * mu0 * s * temp);
+  h = s == 0.0f ? temp : temp / (1.f + 0.5f * mu0 * s * temp);
+  F.f[PIC_EZ][c] = f * F.f[PIC_EZ][c] +
+                   h * (c24 * (27.f * ((by[c + dk] - by[c]) - (bx[c + dl] - bx[c])) -
+                               ((by[c + 2 * dk] - by[c - dk]) - (bx[c + 2 * dl] - bx[c - dl]))) -
+                        mu0 * (F.f[PIC_JIZ][c] - F.f[PIC_JEZ][c]));
+}
+
+// sr_b / sr_e on one rank: the reference copies the total field into the
+// guards.  For B this form stores B1 = B - B0 and B0 is not periodic, so the
+// guard receives B1(src) + B0(src) - B0(guard); E planes pass b0 = nullptr.
+__device__ inline void copy_total(float *f, const float *b0, int64_t dst,
+                                  int64_t src) {
+  f[dst] = b0 ? f[src] + (b0[src] - b0[dst]) : f[src];
+}
+
+__global__ void k_halo_x32(Geom g, float *a, float *b, float *c,
+                           const float *a0, const float *b0, const float *c0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)g.SY * g.SZ;
+  if (i >= n) return;
+  const int64_t dk = n;
+  float *p[3] = {a, b, c};
+  const float *p0[3] = {a0, b0, c0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    copy_total(p[q], p0[q], i, i + g.NX * dk);
+    copy_total(p[q], p0[q], i + dk, i + (g.NX + 1) * dk);
+    copy_total(p[q], p0[q], i + (g.NX + 2) * dk, i + 2 * dk);
+    copy_total(p[q], p0[q], i + (g.NX + 3) * dk, i + 3 * dk);
+  }
+}
+
+__global__ void k_halo_y32(Geom g, float *a, float *b, float *c,
+                           const float *a0, const float *b0, const float *c0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)g.SX * g.SZ;
+  if (i >= n) return;
+  const int k = (int)(i / g.SZ), m = (int)(i % g.SZ);
+  const int64_t dl = g.SZ;
+  const int64_t o = g.idx(k, 0, m);
+  float *p[3] = {a, b, c};
+  const float *p0[3] = {a0, b0, c0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    copy_total(p[q], p0[q], o, o + g.NY * dl);
+    copy_total(p[q], p0[q], o + dl, o + (g.NY + 1) * dl);
+    copy_total(p[q], p0[q], o + (g.NY + 2) * dl, o + 2 * dl);
+    copy_total(p[q], p0[q], o + (g.NY + 3) * dl, o + 3 * dl);
+  }
+}
+
+__global__ void k_fold_x32(Geom g, Fields<float> F) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)g.SY * g.SZ;
+  if (i >= n) return;
+  const int64_t dk = n;
+#pragma unroll
+  for (int q = PIC_JIX; q <= PIC_NE; ++q) {
+    float *f = F.f[q] + i;
+    f[2 * dk] += f[(g.NX + 2) * dk];
+    f[3 * dk] += f[(g.NX + 3) * dk];
+    f[g.NX * dk] += f[0];
+    f[(g.NX + 1) * dk] += f[dk];
+  }
+}
+
+__global__ void k_fold_y32(Geom g, Fields<float> F) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)g.SX * g.SZ;
+  if (i >= n) return;
+  const int k = (int)(i / g.SZ), m = (int)(i % g.SZ);
+  const int64_t dl = g.SZ;
+#pragma unroll
+  for (int q = PIC_JIX; q <= PIC_NE; ++q) {
+    float *f = F.f[q] + g.idx(k, 0, m);
+    f[2 * dl] += f[(g.NY + 2) * dl];
+    f[3 * dl] += f[(g.NY + 3) * dl];
+    f[g.NY * dl] += f[0];
+    f[(g.NY + 1) * dl] += f[dl];
+  }
+}
+
+// deterministic mode: fixed point -> fp32 planes
+__global__ void k_fx_to_f32(const unsigned long long *__restrict__ a, float *b,
+                            int64_t n, double inv_scale) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = (float)((double)(long long)a[i] * inv_scale);
+}
+
+// ---------------------------------------------------------------------------
+// loading, inflow, sort, compaction
+
+__device__ inline void store_draw(Particles32 P, int64_t o, int k, int l,
+                                  int m, const ParticleDraw &d, double vth,
+                                  double mu0, double mu1, double mu2) {
+  float ox = (float)d.ux, oy = (float)d.uy, oz = (float)d.uz;
+  if (ox >= 1.0f) ox = 0.99999994f;
+  if (oy >= 1.0f) oy = 0.99999994f;
+  if (oz >= 1.0f) oz = 0.99999994f;
+  P.cell[o] = pack(k, l, m);
+  P.ox[o] = ox;
+  P.oy[o] = oy;
+  P.oz[o] = oz;
+  P.vx[o] = (float)(vth * d.n0 + mu0);
+  P.vy[o] = (float)(vth * d.n1 + mu1);
+  P.vz[o] = (float)(vth * d.n2 + mu2);
+}
+
+// init_particle_flat (:412-625) with the same Philox streams as the FP64 form
+__global__ void k_load32(Geom g, Particles32 P, int64_t n, int ppc, double vth,
+                         double mu0, double mu1, double mu2, uint64_t seed,
+                         int sp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t cell = i / ppc;
+  const int m = (int)(cell % g.NZ);
+  const int l = (int)((cell / g.NZ) % g.NY);
+  const int k = (int)(cell / ((int64_t)g.NZ * g.NY));
+  store_draw(P, i, k + 2, l + 2, m + 2, draw_particle(seed, sp, 0u, (uint64_t)i),
+             vth, mu0, mu1, mu2);
+}
+
+// add_*_flat refill (:6042-6104): same batch indexing as the FP64 form
+__global__ void k_refill32(Geom g, Particles32 P, int64_t base, int64_t count,
+                           int ppc, double vth, double mu0, double mu1,
+                           double mu2, uint64_t seed, int sp, uint32_t event) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= count) return;
+  const int64_t per_face = (int64_t)g.NX * g.NY * 2 * ppc;
+  const int face = (int)(b / per_face);
+  const int64_t r = b - face * per_face;
+  const int m = (int)((r / ppc) % 2);
+  const int l = (int)((r / (2 * ppc)) % g.NY);
+  const int k = (int)(r / ((int64_t)2 * ppc * g.NY));
+  const int mc = face == 0 ? m + 2 : m + g.NZ;
+  store_draw(P, base + b, k + 2, l + 2, mc,
+             draw_particle(seed, sp, event, (uint64_t)b), vth, mu0, mu1, mu2);
+}
+
+// inflow deletion (:6019-6021): z < 4dx or z > NZ dx, on the cell+offset
+// representation: m + oz < 4 <=> m <= 3; m + oz > NZ <=> m > NZ or
+// (m == NZ and oz > 0)
+__global__ void k_inflow_kill32(Geom g, Particles32 P, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = P.cell[i];
+  if (c < 0) return;
+  const int m = c & kPackMask;
+  if (m <= 3 || m > g.NZ || (m == g.NZ && P.oz[i] > 0.0f)) P.cell[i] = -1;
+}
+
+__global__ void k_live_flags32(const int32_t *__restrict__ cell, int64_t n,
+                               uint8_t *flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = cell[i] >= 0 ? 1 : 0;
+}
+
+__global__ void k_gather32(Particles32 dst, Particles32 src,
+                           const uint32_t *__restrict__ idx, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = idx[i];
+  dst.cell[i] = src.cell[j];
+  dst.ox[i] = src.ox[j];
+  dst.oy[i] = src.oy[j];
+  dst.oz[i] = src.oz[j];
+  dst.vx[i] = src.vx[j];
+  dst.vy[i] = src.vy[j];
+  dst.vz[i] = src.vz[j];
+}
+
+// One add of a workgroup total to a device counter: per-wave ballots and a
+// single atomic per workgroup, so that a counter shared by the whole grid
+// sees gridDim.x atomics instead of one per wave.
+__device__ inline void block_count_add(bool v, unsigned long long *counter) {
+  __shared__ unsigned int part[kBlock / 64];
+  const uint64_t ballot = __ballot(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (unsigned)__popcll(ballot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+    if (t) atomicAdd(counter, (unsigned long long)t);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sort_keys32(
+    Geom g, TileGeom tg, const int32_t *__restrict__ cell, int64_t n,
+    uint32_t *keys, int64_t *valid, uint32_t *tile_count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
+  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
+  uint32_t key = ncell;
+  if (i < n) {
+    const int32_t c = cell[i];
+    if (c >= 0) {
+      int k, l, m;
+      unpack(c, k, l, m);
+      k -= 2;
+      l -= 2;
+      m -= 2;
+      if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
+        key = tg.key(k, l, m);
+    }
+    keys[i] = key;
+  }
+  // per-tile counts: lanes of a wave mostly share a tile (the particles are
+  // nearly sorted), so each run of equal tiles adds once
+  const bool v = i < n && key < ncell;
+  const uint32_t tile = v ? key / cpt : 0xffffffffu;
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(tile, 1);
+  const bool head = v && (lane == 0 || prev != tile);
+  const uint64_t heads = __ballot(head);
+  const uint64_t valid_mask = __ballot(v);
+  if (head) {
+    // the run of this head ends before the next head; count its valid lanes
+    const uint64_t after = lane == 63 ? 0 : (heads >> (lane + 1));
+    const int next = after ? lane + 1 + __ffsll((long long)after) - 1 : 64;
+    const uint64_t span = (next >= 64 ? ~0ull : ((1ull << next) - 1)) &
+                          ~((1ull << lane) - 1);
+    atomicAdd(tile_count + tile, (unsigned)__popcll(valid_mask & span));
+  }
+  block_count_add(v, (unsigned long long *)valid);
+}
+
+// exclusive scan of the per-tile counts in one workgroup: start[t], t<=nt
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *cnt,
+                                                     uint32_t *start, int nt) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (nt + 1023) / 1024;
+  const int lo = t * per, hi = min(nt, lo + per);
+  uint32_t s = 0;
+  for (int i = lo; i < hi; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;
+  for (int i = lo; i < hi; ++i) {
+    start[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) start[nt] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_count_live32(
+    const int32_t *__restrict__ cell, int64_t n, int64_t *count) {
+  unsigned int t = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    t += cell[i] >= 0;
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  __shared__ unsigned int part[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += part[w];
+    if (s) atomicAdd((unsigned long long *)count, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics (fp64 reductions of fp32 state)
+
+constexpr int kNQF = 9, kNQP = 21;
+
+template <int NQ>
+__device__ inline void block_reduce_store(double (&v)[NQ], double *out,
+                                          int max_slot) {
+  __shared__ double sh[NQ][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double a = v[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double b = __shfl_xor(a, o);
+      a = q == max_slot ? fmax(a, b) : a + b;
+    }
+    if (lane == 0) sh[q][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < NQ; ++q) {
+      double a = sh[q][0];
+      for (int j = 1; j < kBlock / 64; ++j)
+        a = q == max_slot ? fmax(a, sh[q][j]) : a + sh[q][j];
+      out[(int64_t)blockIdx.x * NQ + q] = a;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diag_fields32(Geom g, Fields<float> F,
+                                                       double dx, double eps0,
+                                                       double mu0, double q,
+                                                       double *out) {
+  double v[kNQF] = {0};
+  const int64_t ncell = (int64_t)g.NX * g.NY * g.NZ;
+  const double dx3 = dx * dx * dx;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int m = 2 + (int)(t % g.NZ);
+    const int l = 2 + (int)((t / g.NZ) % g.NY);
+    const int k = 2 + (int)(t / ((int64_t)g.NZ * g.NY));
+    const int64_t c = g.idx(k, l, m);
+    const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+    const double ex = F.f[PIC_EX][c], ey = F.f[PIC_EY][c], ez = F.f[PIC_EZ][c];
+    const double b1x = F.f[PIC_BX][c], b1y = F.f[PIC_BY][c], b1z = F.f[PIC_BZ][c];
+    const double bx = b1x + F.f[PIC_B0X][c], by = b1y + F.f[PIC_B0Y][c],
+                 bz = b1z + F.f[PIC_B0Z][c];
+    const double e2 = 0.5 * (ex * ex + ey * ey + ez * ez) * eps0;
+    const double b2 = 0.5 * (bx * bx + by * by + bz * bz) / mu0;
+    v[0] += e2 * dx3;
+    v[1] += b2 * dx3;
+    v[2] += 0.5 * (b1x * b1x + b1y * b1y + b1z * b1z) / mu0 * dx3;
+    const double rho = (double)F.f[PIC_NI][c] - (double)F.f[PIC_NE][c];
+    v[3] += q * rho * dx3;
+    const float *Ex = F.f[PIC_EX], *Ey = F.f[PIC_EY], *Ez = F.f[PIC_EZ];
+    const double r =
+        -q * rho / eps0 +
+        (-(double)Ex[c + 2 * dk] + 27.0 * Ex[c + dk] - 27.0 * Ex[c] + Ex[c - dk] -
+         Ey[c + 2 * dl] + 27.0 * Ey[c + dl] - 27.0 * Ey[c] + Ey[c - dl] -
+         Ez[c + 2] + 27.0 * Ez[c + 1] - 27.0 * Ez[c] + Ez[c - 1]) /
+            (24. * dx);
+    v[4] += r * r;
+    v[5] = fmax(v[5], fabs(r));
+    if (k - 2 < g.NX * 0.8 && k - 2 > g.NX * 0.2 && l - 2 < g.NY * 0.8 &&
+        l - 2 > g.NY * 0.2 && m - 2 < g.NZ * 0.8 && m - 2 > g.NZ * 0.2) {
+      v[7] += e2;
+      v[8] += b2;
+    }
+  }
+  block_reduce_store<kNQF>(v, out, 5);
+}
+
+__global__ __launch_bounds__(256) void k_diag_particles32(
+    Grid32 G, Particles32 P, int64_t n, double dx, double c, double mass,
+    double w, double qs, double *out) {
+  double v[kNQP] = {0};
+  const Geom &g = G.g;
+  const double dx3 = dx * dx * dx;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t cl = P.cell[i];
+    if (cl < 0) continue;
+    int k, l, m;
+    unpack(cl, k, l, m);
+    const double x = (k + (double)P.ox[i]) * dx, y = (l + (double)P.oy[i]) * dx,
+                 z = (m + (double)P.oz[i]) * dx;
+    const double vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double gm = 1.0 / sqrt(1.0 - v2 / (c * c));
+    v[0] += (gm - 1.0) * mass * c * c * w * dx3;
+    v[1] += 0.5 * mass * w * dx3 * v2;
+    v[2] += gm * mass * vx * w * dx3;
+    v[3] += gm * mass * vy * w * dx3;
+    v[4] += gm * mass * vz * w * dx3;
+    if (x < g.NX * dx * 0.8 && x > g.NX * dx * 0.2 && y < g.NY * dx * 0.8 &&
+        y > g.NY * dx * 0.2 && z < g.NZ * dx * 0.8 && z > g.NZ * dx * 0.2 &&
+        k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
+        m <= g.SZ - 2) {
+      float f[6];
+      gather32(G, axis_w(k, P.ox[i]), axis_w(l, P.oy[i]), axis_w(m, P.oz[i]), f);
+      const double qn = qs * w * dx3;
+      v[5] += qn * (f[0] + vy * f[5] - vz * f[4]);
+      v[6] += qn * (f[1] + vz * f[3] - vx * f[5]);
+      v[7] += qn * (f[2] + vx * f[4] - vy * f[3]);
+      v[8] += qn * f[0];
+      v[9] += qn * f[1];
+      v[10] += qn * f[2];
+      v[11] += qn * (vy * f[5] - vz * f[4]);
+      v[12] += qn * (vz * f[3] - vx * f[5]);
+      v[13] += qn * (vx * f[4] - vy * f[3]);
+      v[20] += 0.5 * mass * w * dx3 * v2;
+    }
+  }
+  block_reduce_store<kNQP>(v, out, -1);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct State32 {
+  float *sig32 = nullptr;          // 6 x stride floats
+  uint32_t *tile_count = nullptr;  // ntiles
+  uint32_t *tile_start[2] = {nullptr, nullptr};  // per species, ntiles + 1
+  int64_t sorted_end[2] = {0, 0};
+  unsigned long long *fx = nullptr;  // deterministic accumulators, 8 planes
+  bool fx_dirty = false;
+  double fx_scale = 0;
+};
+
+static State32 *st(pic_ctx *ctx) { return (State32 *)ctx->form_state; }
+
+static Sigma32 sigma_f(pic_ctx *ctx) {
+  Sigma32 s;
+  State32 *S = st(ctx);
+  for (int a = 0; a < 3; ++a) {
+    s.d4[a] = S->sig32 + (2 * a) * ctx->sig_stride;
+    s.d12[a] = S->sig32 + (2 * a + 1) * ctx->sig_stride;
+  }
+  s.amp = (float)ctx->cfg.sigma_amp;
+  return s;
+}
+
+static Grid32 grid32(pic_ctx *ctx) {
+  Fields<float> F = fields<float>(ctx);
+  Grid32 G;
+  G.g = ctx->geom;
+  G.ex = F.f[PIC_EX];
+  G.ey = F.f[PIC_EY];
+  G.ez = F.f[PIC_EZ];
+  G.bx = F.f[PIC_BX];
+  G.by = F.f[PIC_BY];
+  G.bz = F.f[PIC_BZ];
+  G.b0x = F.f[PIC_B0X];
+  G.b0y = F.f[PIC_B0Y];
+  G.b0z = F.f[PIC_B0Z];
+  return G;
+}
+
+int init_state(pic_ctx *ctx) {
+  State32 *S = new (std::nothrow) State32();
+  if (!S) return set_error(ctx, PIC_E_NOMEM, "fp32 state");
+  ctx->form_state = S;
+  const int stride = ctx->sig_stride;
+  float *h = (float *)malloc((size_t)6 * stride * sizeof(float));
+  double *hd = (double *)malloc((size_t)6 * stride * sizeof(double));
+  if (!h || !hd) {
+    free(h);
+    free(hd);
+    return set_error(ctx, PIC_E_NOMEM, "fp32 sigma");
+  }
+  PIC_HIP(ctx, hipMemcpy(hd, ctx->sig_mem, (size_t)6 * stride * sizeof(double),
+                         hipMemcpyDeviceToHost));
+  for (int i = 0; i < 6 * stride; ++i) h[i] = (float)hd[i];
+  free(hd);
+  hipError_t e = hipMalloc(&S->sig32, (size_t)6 * stride * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMemcpy(S->sig32, h, (size_t)6 * stride * sizeof(float),
+                  hipMemcpyHostToDevice);
+  free(h);
+  if (e != hipSuccess)
+    return set_error(ctx, PIC_E_HIP, "fp32 sigma: %s", hipGetErrorString(e));
+  const TileGeom tg = tile_geom(ctx);
+  const int nt = tg.ntiles();
+  PIC_HIP(ctx, hipMalloc(&S->tile_count, (size_t)nt * sizeof(uint32_t)));
+  for (int sp = 0; sp < 2; ++sp) {
+    PIC_HIP(ctx, hipMalloc(&S->tile_start[sp], (size_t)(nt + 1) * sizeof(uint32_t)));
+    PIC_HIP(ctx, hipMemset(S->tile_start[sp], 0, (size_t)(nt + 1) * sizeof(uint32_t)));
+  }
+  {
+    // fixed-point unit: 2^-40 of the largest single contribution q w c/dx^3
+    // (per-node sums of a few thousand contributions stay far below 2^63)
+    const pic_config &c = ctx->cfg;
+    const double wmax = c.weight[0] > c.weight[1] ? c.weight[0] : c.weight[1];
+    const double qmax = c.q > 1.0 ? c.q : 1.0;
+    const double unit = qmax * wmax * (c.c > 1.0 ? c.c : 1.0) /
+                        (c.dx * c.dx * c.dx);
+    S->fx_scale = ldexp(1.0, 40) / unit;
+  }
+  if (ctx->cfg.deposit == PIC_DEPOSIT_DETERMINISTIC) {
+    PIC_HIP(ctx, hipMalloc(&S->fx, (size_t)8 * ctx->geom.plane * sizeof(unsigned long long)));
+    PIC_HIP(ctx, hipMemset(S->fx, 0, (size_t)8 * ctx->geom.plane * sizeof(unsigned long long)));
+  }
+  return 0;
+}
+
+void free_state(pic_ctx *ctx) {
+  State32 *S = st(ctx);
+  if (!S) return;
+  if (S->sig32) (void)hipFree(S->sig32);
+  if (S->tile_count) (void)hipFree(S->tile_count);
+  for (int sp = 0; sp < 2; ++sp)
+    if (S->tile_start[sp]) (void)hipFree(S->tile_start[sp]);
+  if (S->fx) (void)hipFree(S->fx);
+  delete S;
+  ctx->form_state = nullptr;
+}
+
+static dim3 interior_grid(const Geom &g, dim3 &block) {
+  block = dim3(64, 4, 1);
+  return dim3((unsigned)div_up(g.NZ, 64), (unsigned)div_up(g.NY, 4),
+              (unsigned)g.NX);
+}
+
+int renew_grid(pic_ctx *ctx) {
+  Fields<float> F = fields<float>(ctx);
+  k_zero32<<<2048, kBlock, 0, ctx->stream>>>(F.f[PIC_JIX], 8 * ctx->geom.plane);
+  State32 *S = st(ctx);
+  if (S->fx) {
+    k_zero64<<<2048, kBlock, 0, ctx->stream>>>(S->fx, 8 * ctx->geom.plane);
+    S->fx_dirty = false;
+  }
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int finish_deposit(pic_ctx *ctx) {
+  State32 *S = st(ctx);
+  if (!S || !S->fx || !S->fx_dirty) return 0;
+  Fields<float> F = fields<float>(ctx);
+  k_fx_to_f32<<<2048, kBlock, 0, ctx->stream>>>(S->fx, F.f[PIC_JIX],
+                                                8 * ctx->geom.plane,
+                                                1.0 / S->fx_scale);
+  PIC_LAUNCH_CHECK(ctx);
+  S->fx_dirty = false;
+  return 0;
+}
+
+int renew_b_half(pic_ctx *ctx) {
+  const pic_config &c = ctx->cfg;
+  dim3 b, gr = interior_grid(ctx->geom, b);
+  k_renew_b32<<<gr, b, 0, ctx->stream>>>(ctx->geom, fields<float>(ctx),
+                                         sigma_f(ctx), (float)c.dt,
+                                         (float)c.dx, (float)c.eps0);
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int renew_e(pic_ctx *ctx) {
+  const pic_config &c = ctx->cfg;
+  dim3 b, gr = interior_grid(ctx->geom, b);
+  k_renew_e32<<<gr, b, 0, ctx->stream>>>(ctx->geom, fields<float>(ctx),
+                                         sigma_f(ctx), (float)c.dt,
+                                         (float)c.dx, (float)c.c, (float)c.mu0);
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int halo(pic_ctx *ctx, int first) {
+  const Geom &g = ctx->geom;
+  Fields<float> F = fields<float>(ctx);
+  float *a = F.f[first], *b = F.f[first + 1], *c = F.f[first + 2];
+  const bool isb = first == PIC_BX;
+  const float *a0 = isb ? F.f[PIC_B0X] : nullptr, *b0 = isb ? F.f[PIC_B0Y] : nullptr,
+              *c0 = isb ? F.f[PIC_B0Z] : nullptr;
+  k_halo_x32<<<(unsigned)div_up((int64_t)g.SY * g.SZ, kBlock), kBlock, 0,
+               ctx->stream>>>(g, a, b, c, a0, b0, c0);
+  k_halo_y32<<<(unsigned)div_up((int64_t)g.SX * g.SZ, kBlock), kBlock, 0,
+               ctx->stream>>>(g, a, b, c, a0, b0, c0);
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int fold_current(pic_ctx *ctx) {
+  const Geom &g = ctx->geom;
+  Fields<float> F = fields<float>(ctx);
+  k_fold_x32<<<(unsigned)div_up((int64_t)g.SY * g.SZ, kBlock), kBlock, 0,
+               ctx->stream>>>(g, F);
+  k_fold_y32<<<(unsigned)div_up((int64_t)g.SX * g.SZ, kBlock), kBlock, 0,
+               ctx->stream>>>(g, F);
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int load_particles(pic_ctx *ctx) {
+  const pic_config &c = ctx->cfg;
+  const int64_t n = (int64_t)c.nx * c.ny * c.nz * c.ppc;
+  if (n > ctx->cap)
+    return set_error(ctx, PIC_E_CAPACITY, "load: %lld > capacity %lld",
+                     (long long)n, (long long)ctx->cap);
+  const TileGeom tg = tile_geom(ctx);
+  const bool ref_order = tg.ntiles() == 1;
+  for (int sp = 0; sp < 2; ++sp) {
+    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+    if (n > 0)
+      k_load32<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
+          ctx->geom, P, n, c.ppc, c.vth[sp], c.vdrift[0], c.vdrift[1],
+          c.vdrift[2], c.seed, sp);
+    PIC_LAUNCH_CHECK(ctx);
+    ctx->n[sp] = n;
+    st(ctx)->sorted_end[sp] = 0;
+    (void)ref_order;
+  }
+  // the loaded order is the reference order; bring it to tile order
+  for (int sp = 0; sp < 2; ++sp) {
+    int rc = sort(ctx, sp);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+static Push32 push_const(const pic_ctx *ctx, int sp) {
+  const pic_config &c = ctx->cfg;
+  Push32 p;
+  const double m = sp == PIC_ION ? c.mass[0] : -c.mass[1];
+  p.T = (float)(0.5 * c.q * c.dt / m);
+  p.hdt = (float)(0.5 * c.dt / c.dx);
+  p.c = (float)c.c;
+  p.c2 = (float)(c.c * c.c);
+  const double dx3 = c.dx * c.dx * c.dx;
+  p.qw = (float)(c.q * c.weight[sp] / dx3);
+  p.w = (float)(c.weight[sp] / dx3);
+  p.eimf_x = (float)(c.vdrift[2] * c.imf[1]);
+  p.eimf_y = (float)(-c.vdrift[2] * c.imf[0]);
+  for (int a = 0; a < 3; ++a) {
+    p.blo[a] = (float)(c.ballistic_lo[a] / c.dx);
+    p.bhi[a] = (float)(c.ballistic_hi[a] / c.dx);
+  }
+  p.periodic[0] = c.periodic[0];
+  p.periodic[1] = c.periodic[1];
+  p.fx_scale = ctx->form_state ? ((State32 *)ctx->form_state)->fx_scale : 0.0;
+  return p;
+}
+
+int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
+         int do_boundary) {
+  const int64_t n = ctx->n[sp];
+  if (n == 0) return 0;
+  State32 *S = st(ctx);
+  const TileGeom tg = tile_geom(ctx);
+  TileRun tr;
+  tr.tile_start = S->tile_start[sp];
+  tr.sorted_end = S->sorted_end[sp];
+  tr.ntiles = tr.sorted_end > 0 ? tg.ntiles() : 0;
+  if (tr.ntiles == 0) tr.sorted_end = 0;
+  tr.n = n;
+  tr.ntail = (int)div_up(n - tr.sorted_end, kTailChunk);
+  const unsigned nb = (unsigned)(tr.ntiles + tr.ntail);
+  if (nb == 0) return 0;
+  Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+  Push32 kp = push_const(ctx, sp);
+  Grid32 G = grid32(ctx);
+  const bool det = S->fx != nullptr;
+  const int sel = (do_push_j ? 4 : 0) | (do_drift_n ? 2 : 0) | (do_boundary ? 1 : 0);
+  if (det) {
+    unsigned long long *base = S->fx;
+    const int64_t pl = ctx->geom.plane;
+    const int off = sp == PIC_ION ? 0 : 3;
+    unsigned long long *jx = base + (off + 0) * pl, *jy = base + (off + 1) * pl,
+                       *jz = base + (off + 2) * pl,
+                       *nd = base + (6 + (sp == PIC_ION ? 0 : 1)) * pl;
+#define L(A, B, C) \
+  k_particles32<A, B, C, true><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd)
+    switch (sel) {
+      case 7: L(true, true, true); break;
+      case 6: L(true, true, false); break;
+      case 4: L(true, false, false); break;
+      case 2: L(false, true, false); break;
+      case 1: L(false, false, true); break;
+      case 3: L(false, true, true); break;
+      case 5: L(true, false, true); break;
+      default: return 0;
+    }
+#undef L
+    if (do_push_j || do_drift_n) S->fx_dirty = true;
+  } else {
+    Fields<float> F = fields<float>(ctx);
+    float *jx = F.f[sp == PIC_ION ? PIC_JIX : PIC_JEX];
+    float *jy = F.f[sp == PIC_ION ? PIC_JIY : PIC_JEY];
+    float *jz = F.f[sp == PIC_ION ? PIC_JIZ : PIC_JEZ];
+    float *nd = F.f[sp == PIC_ION ? PIC_NI : PIC_NE];
+#define L(A, B, C) \
+  k_particles32<A, B, C, false><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd)
+    switch (sel) {
+      case 7: L(true, true, true); break;
+      case 6: L(true, true, false); break;
+      case 4: L(true, false, false); break;
+      case 2: L(false, true, false); break;
+      case 1: L(false, false, true); break;
+      case 3: L(false, true, true); break;
+      case 5: L(true, false, true); break;
+      default: return 0;
+    }
+#undef L
+  }
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+int particle_boundary(pic_ctx *ctx, int sp) { return push(ctx, sp, 0, 0, 1); }
+
+// stable compaction of live particles (tile ranges become invalid)
+static int compact(pic_ctx *ctx, int sp) {
+  const int64_t n = ctx->n[sp];
+  Particles32 src = p32(ctx, sp, ctx->cur[sp]);
+  k_live_flags32<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
+      src.cell, n, ctx->flags);
+  PIC_LAUNCH_CHECK(ctx);
+  int64_t kept = 0;
+  int rc = select_flagged(ctx, ctx->flags, ctx->idx, n, &kept);
+  if (rc) return rc;
+  Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
+  if (kept > 0)
+    k_gather32<<<(unsigned)div_up(kept, kBlock), kBlock, 0, ctx->stream>>>(
+        dst, src, ctx->idx, kept);
+  PIC_LAUNCH_CHECK(ctx);
+  ctx->cur[sp] = 1 - ctx->cur[sp];
+  ctx->n[sp] = kept;
+  st(ctx)->sorted_end[sp] = 0;
+  return 0;
+}
+
+// add_*_flat: slab particles become dead slots in place, new particles are
+// appended after the stored ones (the sort compacts)
+int inflow(pic_ctx *ctx, int sp, uint32_t event) {
+  const pic_config &c = ctx->cfg;
+  const Geom &g = ctx->geom;
+  if (ctx->n[sp] > 0) {
+    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+    k_inflow_kill32<<<(unsigned)div_up(ctx->n[sp], kBlock), kBlock, 0,
+                      ctx->stream>>>(g, P, ctx->n[sp]);
+    PIC_LAUNCH_CHECK(ctx);
+  }
+  const int nface = c.refill_far ? 2 : 1;
+  const int64_t add = (int64_t)g.NX * g.NY * 2 * c.ppc * nface;
+  if (ctx->n[sp] + add > ctx->cap) {
+    int rc = compact(ctx, sp);
+    if (rc) return rc;
+  }
+  if (ctx->n[sp] + add > ctx->cap)
+    return set_error(ctx, PIC_E_CAPACITY,
+                     "inflow: %lld + %lld particles exceed capacity %lld",
+                     (long long)ctx->n[sp], (long long)add, (long long)ctx->cap);
+  if (add > 0) {
+    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+    k_refill32<<<(unsigned)div_up(add, kBlock), kBlock, 0, ctx->stream>>>(
+        g, P, ctx->n[sp], add, c.ppc, c.vth[sp], c.vdrift[0], c.vdrift[1],
+        c.vdrift[2], c.seed, sp, event);
+    PIC_LAUNCH_CHECK(ctx);
+  }
+  ctx->n[sp] += add;
+  return 0;
+}
+
+int sort(pic_ctx *ctx, int sp) {
+  const Geom &g = ctx->geom;
+  State32 *S = st(ctx);
+  const TileGeom tg = tile_geom(ctx);
+  const int nt = tg.ntiles();
+  const int64_t n = ctx->n[sp];
+  if (n == 0) {
+    S->sorted_end[sp] = 0;
+    return 0;
+  }
+  Particles32 src = p32(ctx, sp, ctx->cur[sp]);
+  PIC_HIP(ctx, hipMemsetAsync(ctx->dev_counter, 0, sizeof(int64_t), ctx->stream));
+  PIC_HIP(ctx, hipMemsetAsync(S->tile_count, 0, (size_t)nt * sizeof(uint32_t),
+                              ctx->stream));
+  k_sort_keys32<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
+      g, tg, src.cell, n, ctx->keys, ctx->dev_counter, S->tile_count);
+  PIC_LAUNCH_CHECK(ctx);
+  const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
+  int bits = 1;
+  while (bits < 32 && ((uint64_t)1 << bits) <= ncell) ++bits;
+  int rc = sort_keys(ctx, ctx->keys, ctx->keys_alt, ctx->idx, n, bits);
+  if (rc) return rc;
+  k_scan_tiles<<<1, 1024, 0, ctx->stream>>>(S->tile_count, S->tile_start[sp], nt);
+  PIC_LAUNCH_CHECK(ctx);
+  PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->dev_counter,
+                              sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int64_t valid = ctx->host_counter[0];
+  Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
+  if (valid > 0)
+    k_gather32<<<(unsigned)div_up(valid, kBlock), kBlock, 0, ctx->stream>>>(
+        dst, src, ctx->idx, valid);
+  PIC_LAUNCH_CHECK(ctx);
+  ctx->cur[sp] = 1 - ctx->cur[sp];
+  ctx->n[sp] = valid;
+  S->sorted_end[sp] = valid;
+  return 0;
+}
+
+int count_live(pic_ctx *ctx, int sp, int64_t *out) {
+  const int64_t n = ctx->n[sp];
+  *out = 0;
+  if (n == 0) return 0;
+  Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+  PIC_HIP(ctx, hipMemsetAsync(ctx->dev_counter, 0, sizeof(int64_t), ctx->stream));
+  k_count_live32<<<(unsigned)std::min<int64_t>(2048, div_up(n, kBlock)), kBlock,
+                   0, ctx->stream>>>(P.cell, n, ctx->dev_counter);
+  PIC_LAUNCH_CHECK(ctx);
+  PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->dev_counter,
+                              sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = ctx->host_counter[0];
+  return 0;
+}
+
+int diagnose(pic_ctx *ctx, pic_diag *out) {
+  const pic_config &c = ctx->cfg;
+  const Geom &g = ctx->geom;
+  const int nbf = 1024, nbp = 2048;
+  const int64_t need = (int64_t)nbf * kNQF + 2 * (int64_t)nbp * kNQP;
+  if (ctx->diag_partial_cap < need) {
+    if (ctx->diag_partial) (void)hipFree(ctx->diag_partial);
+    ctx->diag_partial = nullptr;
+    PIC_HIP(ctx, hipMalloc(&ctx->diag_partial, need * sizeof(double)));
+    ctx->diag_partial_cap = need;
+  }
+  double *pf = ctx->diag_partial;
+  double *pp[2] = {pf + (int64_t)nbf * kNQF,
+                   pf + (int64_t)nbf * kNQF + (int64_t)nbp * kNQP};
+  k_diag_fields32<<<nbf, kBlock, 0, ctx->stream>>>(g, fields<float>(ctx), c.dx,
+                                                   c.eps0, c.mu0, c.q, pf);
+  PIC_LAUNCH_CHECK(ctx);
+  Grid32 G = grid32(ctx);
+  for (int sp = 0; sp < 2; ++sp) {
+    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+    k_diag_particles32<<<nbp, kBlock, 0, ctx->stream>>>(
+        G, P, ctx->n[sp], c.dx, c.c, c.mass[sp], c.weight[sp],
+        sp == PIC_ION ? c.q : -c.q, pp[sp]);
+    PIC_LAUNCH_CHECK(ctx);
+  }
+  double *h = (double *)malloc(need * sizeof(double));
+  if (!h) return set_error(ctx, PIC_E_NOMEM, "diagnose: host buffer");
+  hipError_t e = hipMemcpyAsync(h, pf, need * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free(h);
+    return set_error(ctx, PIC_E_HIP, "diagnose: %s", hipGetErrorString(e));
+  }
+  memset(out, 0, sizeof(*out));
+  double s[kNQF] = {0};
+  for (int b = 0; b < nbf; ++b)
+    for (int q = 0; q < kNQF; ++q) {
+      const double v = h[(int64_t)b * kNQF + q];
+      s[q] = q == 5 ? fmax(s[q], v) : s[q] + v;
+    }
+  out->field_energy_e = s[0];
+  out->field_energy_b = s[1];
+  out->field_energy_b1 = s[2];
+  out->charge = s[3];
+  out->gauss_l2 = sqrt(s[4]);
+  out->gauss_max = s[5];
+  out->energy_box[0] = s[7];
+  out->energy_box[1] = s[8];
+  for (int sp = 0; sp < 2; ++sp) {
+    double t[kNQP] = {0};
+    const double *hp = h + (pp[sp] - pf);
+    for (int b = 0; b < nbp; ++b)
+      for (int q = 0; q < kNQP; ++q) t[q] += hp[(int64_t)b * kNQP + q];
+    out->kinetic[sp] = t[0];
+    out->kinetic_nr[sp] = t[1];
+    for (int j = 0; j < 3; ++j) out->momentum[sp][j] = t[2 + j];
+    for (int j = 0; j < 3; ++j) out->force[j] += t[5 + j];
+    const int b0 = sp == PIC_ION ? 3 : 9;
+    for (int j = 0; j < 6; ++j) out->force[b0 + j] = t[8 + j];
+    out->energy_box[2 + sp] = t[20];
+  }
+  free(h);
+  for (int sp = 0; sp < 2; ++sp) {
+    int rc = count_live(ctx, sp, &out->count[sp]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int upload_particles(pic_ctx *ctx, int sp, int64_t n, const double *const *h) {
+  const double dx = ctx->cfg.dx;
+  const Geom &g = ctx->geom;
+  const int64_t nn = n > 0 ? n : 1;
+  int32_t *cell = (int32_t *)malloc(nn * sizeof(int32_t));
+  float *f = (float *)malloc(nn * 6 * sizeof(float));
+  if (!cell || !f) {
+    free(cell);
+    free(f);
+    return set_error(ctx, PIC_E_NOMEM, "upload: host buffer");
+  }
+  const int S[3] = {g.SX, g.SY, g.SZ};
+  for (int64_t i = 0; i < n; ++i) {
+    int kc[3];
+    float o[3];
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      const double u = h[a][i] / dx;
+      if (!(u >= 0.0 && u < S[a])) {
+        ok = false;
+        break;
+      }
+      kc[a] = (int)u;
+      o[a] = (float)(u - kc[a]);
+      if (o[a] >= 1.0f) o[a] = 0.99999994f;
+    }
+    cell[i] = ok ? pack(kc[0], kc[1], kc[2]) : -1;
+    for (int a = 0; a < 3; ++a) f[a * nn + i] = ok ? o[a] : 0.0f;
+    for (int a = 0; a < 3; ++a) f[(3 + a) * nn + i] = (float)h[3 + a][i];
+  }
+  Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+  float *d[6] = {P.ox, P.oy, P.oz, P.vx, P.vy, P.vz};
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && n > 0)
+    e = hipMemcpy(P.cell, cell, n * sizeof(int32_t), hipMemcpyHostToDevice);
+  for (int a = 0; a < 6 && e == hipSuccess && n > 0; ++a)
+    e = hipMemcpy(d[a], f + a * nn, n * sizeof(float), hipMemcpyHostToDevice);
+  free(cell);
+  free(f);
+  if (e != hipSuccess)
+    return set_error(ctx, PIC_E_HIP, "upload: %s", hipGetErrorString(e));
+  ctx->n[sp] = n;
+  st(ctx)->sorted_end[sp] = 0;
+  return 0;
+}
+
+static int download_raw(pic_ctx *ctx, int sp, int32_t **cell, float **f,
+                        int64_t *ns) {
+  *ns = ctx->n[sp];
+  const int64_t nn = *ns > 0 ? *ns : 1;
+  *cell = (int32_t *)malloc(nn * sizeof(int32_t));
+  *f = (float *)malloc(nn * 6 * sizeof(float));
+  if (!*cell || !*f) {
+    free(*cell);
+    free(*f);
+    return set_error(ctx, PIC_E_NOMEM, "download: host buffer");
+  }
+  Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+  const float *d[6] = {P.ox, P.oy, P.oz, P.vx, P.vy, P.vz};
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && *ns > 0)
+    e = hipMemcpy(*cell, P.cell, *ns * sizeof(int32_t), hipMemcpyDeviceToHost);
+  for (int a = 0; a < 6 && e == hipSuccess && *ns > 0; ++a)
+    e = hipMemcpy(*f + a * nn, d[a], *ns * sizeof(float), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    free(*cell);
+    free(*f);
+    return set_error(ctx, PIC_E_HIP, "download: %s", hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int download_particles(pic_ctx *ctx, int sp, int64_t cap, int64_t *n,
+                       double *const *h) {
+  int32_t *cell;
+  float *f;
+  int64_t ns;
+  int rc = download_raw(ctx, sp, &cell, &f, &ns);
+  if (rc) return rc;
+  const int64_t nn = ns > 0 ? ns : 1;
+  int64_t live = 0;
+  for (int64_t i = 0; i < ns; ++i) live += cell[i] >= 0;
+  *n = live;
+  if (cap < live) {
+    free(cell);
+    free(f);
+    return set_error(ctx, PIC_E_CAPACITY, "download: %lld live > cap %lld",
+                     (long long)live, (long long)cap);
+  }
+  const double dx = ctx->cfg.dx;
+  int64_t w = 0;
+  for (int64_t i = 0; i < ns; ++i) {
+    const int32_t c = cell[i];
+    if (c < 0) continue;
+    const int k = c >> 20, l = (c >> 10) & kPackMask, m = c & kPackMask;
+    h[0][w] = (k + (double)f[0 * nn + i]) * dx;
+    h[1][w] = (l + (double)f[1 * nn + i]) * dx;
+    h[2][w] = (m + (double)f[2 * nn + i]) * dx;
+    for (int a = 0; a < 3; ++a) h[3 + a][w] = f[(3 + a) * nn + i];
+    w++;
+  }
+  free(cell);
+  free(f);
+  return 0;
+}
+
+int download_raw32(pic_ctx *ctx, int sp, int64_t cap, int64_t *n,
+                   int32_t *cell_out, float *const *fo) {
+  int32_t *cell;
+  float *f;
+  int64_t ns;
+  int rc = download_raw(ctx, sp, &cell, &f, &ns);
+  if (rc) return rc;
+  const int64_t nn = ns > 0 ? ns : 1;
+  int64_t live = 0;
+  for (int64_t i = 0; i < ns; ++i) live += cell[i] >= 0;
+  *n = live;
+  if (cap < live) {
+    free(cell);
+    free(f);
+    return set_error(ctx, PIC_E_CAPACITY, "download: %lld live > cap %lld",
+                     (long long)live, (long long)cap);
+  }
+  int64_t w = 0;
+  for (int64_t i = 0; i < ns; ++i) {
+    if (cell[i] < 0) continue;
+    cell_out[w] = cell[i];
+    for (int a = 0; a < 6; ++a) fo[a][w] = f[a * nn + i];
+    w++;
+  }
+  free(cell);
+  free(f);
+  return 0;
+}
+
 }  // namespace f32
 }  // namespace pic
 
-extern "C" int pic_download_particles_raw32(pic_ctx *ctx, int, int64_t,
-                                            int64_t *, int32_t *, float *,
-                                            float *, float *, float *, float *,
-                                            float *) {
-  return pic::set_error(ctx, PIC_E_STATE, "FP32 form not built");
+extern "C" int pic_download_particles_raw32(pic_ctx *ctx, int species,
+                                            int64_t cap, int64_t *n,
+                                            int32_t *cell, float *ox, float *oy,
+                                            float *oz, float *vx, float *vy,
+                                            float *vz) {
+  if (!ctx || !n || species < 0 || species > 1) return PIC_E_ARG;
+  if (ctx->cfg.precision != PIC_FP32)
+    return pic::set_error(ctx, PIC_E_STATE, "raw32 download needs the FP32 form");
+  float *const f[6] = {ox, oy, oz, vx, vy, vz};
+  return pic::f32::download_raw32(ctx, species, cap, n, cell, f);
 }

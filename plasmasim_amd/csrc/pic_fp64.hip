@@ -564,7 +564,7 @@ __global__ void k_gather_particles(Particles64 dst, Particles64 src,
 // sorting key (:7488-7497): ((int)(x/dx)-2, (int)(y/dx)-2, (int)(z/dx)-2),
 // lexicographic with m fastest; particles outside the interior (and dead
 // ones) get the key ncell and are dropped after the sort.
-__global__ void k_sort_keys(Geom g, TileGeom tg, Particles64 P, int64_t n,
+__global__ __launch_bounds__(256) void k_sort_keys(Geom g, TileGeom tg, Particles64 P, int64_t n,
                             double dx, uint32_t *keys, int64_t *valid) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
@@ -582,20 +582,34 @@ __global__ void k_sort_keys(Geom g, TileGeom tg, Particles64 P, int64_t n,
     }
     keys[i] = key;
   }
-  // count valid keys with one atomic per wavefront
+  // count valid keys: per-wave ballots, one atomic per workgroup
   const bool v = i < n && key < ncell;
+  __shared__ unsigned int part[kBlock / 64];
   const uint64_t ballot = __ballot(v);
-  if ((threadIdx.x & 63) == 0 && ballot)
-    atomicAdd((unsigned long long *)valid, (unsigned long long)__popcll(ballot));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (unsigned)__popcll(ballot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+    if (t) atomicAdd((unsigned long long *)valid, (unsigned long long)t);
+  }
 }
 
-__global__ void k_count_live(const double *__restrict__ z, int64_t n,
-                             int64_t *count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool v = i < n && !isnan(z[i]);
-  const uint64_t ballot = __ballot(v);
-  if ((threadIdx.x & 63) == 0 && ballot)
-    atomicAdd((unsigned long long *)count, (unsigned long long)__popcll(ballot));
+__global__ __launch_bounds__(256) void k_count_live(
+    const double *__restrict__ z, int64_t n, int64_t *count) {
+  unsigned int t = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    t += !isnan(z[i]);
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  __shared__ unsigned int part[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += part[w];
+    if (s) atomicAdd((unsigned long long *)count, s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -911,8 +925,8 @@ int count_live(pic_ctx *ctx, int sp, int64_t *out) {
   if (n == 0) return 0;
   Particles64 P = p64(ctx, sp, ctx->cur[sp]);
   PIC_HIP(ctx, hipMemsetAsync(ctx->dev_counter, 0, sizeof(int64_t), ctx->stream));
-  k_count_live<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
-      P.z, n, ctx->dev_counter);
+  k_count_live<<<(unsigned)(div_up(n, kBlock) < 2048 ? div_up(n, kBlock) : 2048),
+                 kBlock, 0, ctx->stream>>>(P.z, n, ctx->dev_counter);
   PIC_LAUNCH_CHECK(ctx);
   PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->dev_counter,
                               sizeof(int64_t), hipMemcpyDeviceToHost,

@@ -104,6 +104,10 @@ struct pic_ctx {
   double *diag_partial = nullptr;  // reduction partials
   int64_t diag_partial_cap = 0;
 
+  // state private to the numeric form (FP32: sigma copy, tile ranges,
+  // deterministic accumulators)
+  void *form_state = nullptr;
+
   // step counters (reference fdtd(): c, c_add, c_thrust)
   int64_t step = 0;
   int32_t c_add = 0, c_sort = 0;
@@ -213,6 +217,10 @@ int upload_particles(pic_ctx *, int sp, int64_t n, const double *const *h);
 int download_particles(pic_ctx *, int sp, int64_t cap, int64_t *n,
                        double *const *h);
 int finish_deposit(pic_ctx *);
+int init_state(pic_ctx *);
+void free_state(pic_ctx *);
+int download_raw32(pic_ctx *, int sp, int64_t cap, int64_t *n, int32_t *cell,
+                   float *const *f);
 }  // namespace f32
 
 // device-wide building blocks on rocPRIM (pic_prim.hip)

@@ -1,0 +1,202 @@
+"""The FP32 chosen form against the FP64 plain form and the oracle.
+
+Bars (tolerances stated per assertion):
+  - loaded/inflow positions: cell exact, offset within fp32 rounding;
+  - tile-order sort: same permutation as the oracle's stable sort (integer);
+  - one step from an identical developed state: relative L2 of J, E and the
+    induced field B - B0 against the FP64 form <= 1e-5;
+  - deterministic deposit: bit-identical results run to run.
+"""
+import numpy as np
+import pytest
+
+import plasmasim_amd as ps
+from helpers import rel_l2
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+FIELDS = ["ex", "ey", "ez", "bx", "by", "bz"]
+J = ["jix", "jiy", "jiz", "jex", "jey", "jez", "ni", "ne"]
+
+
+def cfg(preset=1, precision=ps.PIC_FP32, tile=(4, 4, 8), deposit=ps.PIC_DEPOSIT_ATOMIC):
+    c = ps.preset(preset)
+    c.precision = precision
+    c.tile[0], c.tile[1], c.tile[2] = tile
+    c.deposit = deposit
+    return c
+
+
+def test_load_matches_fp64_stream():
+    c = cfg()
+    g = ps.Simulation(c)
+    g.load_particles()
+    o = Oracle(cfg(precision=ps.PIC_FP64))
+    o.load_particles()
+    o.sort(0)
+    o.sort(1)
+    for sp in (0, 1):
+        a, b = g.download_particles(sp), o.download_particles(sp)
+        assert a.shape == b.shape
+        assert np.array_equal(np.floor(a[:, :3]), np.floor(b[:, :3]))
+        assert np.abs(a[:, :3] - b[:, :3]).max() < 1.2e-7 * 2
+        assert np.abs(a[:, 3:] - b[:, 3:]).max() <= 6e-8 * np.abs(b[:, 3:]).max()
+
+
+def test_tile_sort_permutation_matches_oracle():
+    c = cfg(1)
+    g = ps.Simulation(c)
+    c64 = cfg(1, precision=ps.PIC_FP64)
+    o = Oracle(c64)
+    rng = np.random.default_rng(3)
+    n = 300000
+    p = np.column_stack([rng.uniform(0.5, c.nx + 3.5, n), rng.uniform(0.5, c.ny + 3.5, n),
+                         rng.uniform(0.5, c.nz + 3.5, n), rng.standard_normal((n, 3)) * 0.01])
+    # snap positions to fp32-representable offsets so both sides hold the
+    # same particle and the same cell
+    cell = np.floor(p[:, :3])
+    p[:, :3] = cell + np.minimum(np.float32(p[:, :3] - cell).astype(np.float64), 0.99999994)
+    p[:, 3:] = p[:, 3:].astype(np.float32)
+    g.upload_particles(1, p)
+    o.upload_particles(1, p)
+    g.sort(1)
+    o.sort(1)
+    a, b = g.download_particles(1), o.download_particles(1)
+    assert a.shape == b.shape
+    assert np.array_equal(a, b)
+
+
+def sims_with_state(f, p=None):
+    out = []
+    for prec in (ps.PIC_FP32, ps.PIC_FP64):
+        s = ps.Simulation(cfg(1, precision=prec))
+        for n, a in f.items():
+            s.upload_field(n, a)
+        if p is not None:
+            for sp in (0, 1):
+                s.upload_particles(sp, p[sp])
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("routine", ["renew_b_half", "renew_e", "halo_b", "halo_e",
+                                     "fold_current"])
+def test_field_routine_vs_fp64(routine):
+    c = cfg(1, precision=ps.PIC_FP64)
+    sh = (c.nx + 4, c.ny + 4, c.nz + 4)
+    rng = np.random.default_rng(21)
+    probe = ps.Simulation(c)
+    b0 = [probe.download_field(n) for n in ("b0x", "b0y", "b0z")]
+    f = {n: rng.standard_normal(sh) * 1e-2 for n in ("ex", "ey", "ez")}
+    for i, n in enumerate(("bx", "by", "bz")):
+        f[n] = b0[i] + rng.standard_normal(sh) * 1e-2
+    for n in J:
+        f[n] = rng.standard_normal(sh) * 1e-3
+    s32, s64 = sims_with_state(f)
+    getattr(s32, routine)()
+    getattr(s64, routine)()
+    for n in ("ex", "ey", "ez") + tuple(J):
+        e = rel_l2(s32.download_field(n), s64.download_field(n))
+        assert e < 3e-7, (routine, n, e)
+    for i, n in enumerate(("bx", "by", "bz")):
+        e = rel_l2(s32.download_field(n) - b0[i], s64.download_field(n) - b0[i])
+        assert e < 3e-7, (routine, n, e)
+
+
+@pytest.mark.parametrize("sp", [0, 1])
+@pytest.mark.parametrize("routine", ["push_deposit_j", "drift_deposit_n"])
+def test_particle_routine_vs_fp64(state, routine, sp):
+    f, p = state
+    s32, s64 = sims_with_state(f, p)
+    for s in (s32, s64):
+        s.renew_grid()
+        getattr(s, routine)(sp)
+    errs = {}
+    for n in J:
+        a, b = s32.download_field(n), s64.download_field(n)
+        if np.linalg.norm(b) == 0:
+            continue
+        errs[n] = rel_l2(a, b)
+    print(routine, sp, {k: f"{v:.1e}" for k, v in errs.items()})
+    for n, e in errs.items():
+        assert e < 1e-5, (routine, sp, n, e)
+    a, b = s32.download_particles(sp), s64.download_particles(sp)
+    assert a.shape == b.shape
+    assert np.abs(a[:, :3] - b[:, :3]).max() < 1e-5
+    assert rel_l2(a[:, 3:], b[:, 3:]) < 1e-6
+
+
+def developed_state(steps=20):
+    """Oracle state after `steps` FP64 steps of the 32^3 workload."""
+    c64 = cfg(1, precision=ps.PIC_FP64)
+    o = Oracle(c64)
+    o.load_particles()
+    o.begin()
+    o.step(steps)
+    f = {n: o.download_field(n) for n in FIELDS}
+    p = [o.download_particles(sp) for sp in (0, 1)]
+    return f, p
+
+
+@pytest.fixture(scope="module")
+def state():
+    return developed_state()
+
+
+def run_one(precision, f, p, deposit=ps.PIC_DEPOSIT_ATOMIC):
+    c = cfg(1, precision=precision, deposit=deposit)
+    s = ps.Simulation(c)
+    for n, a in f.items():
+        s.upload_field(n, a)
+    for sp in (0, 1):
+        s.upload_particles(sp, p[sp])
+    s.step(1)
+    out = {n: s.download_field(n) for n in FIELDS + J}
+    out["b0"] = [s.download_field(n) for n in ("b0x", "b0y", "b0z")]
+    out["count"] = [s.particle_count(sp) for sp in (0, 1)]
+    s.close()
+    return out
+
+
+def test_one_step_error_vs_fp64(state):
+    f, p = state
+    r64 = run_one(ps.PIC_FP64, f, p)
+    r32 = run_one(ps.PIC_FP32, f, p)
+    errs = {}
+    for n in J + ["ex", "ey", "ez"]:
+        errs[n] = rel_l2(r32[n], r64[n])
+    for i, n in enumerate(("bx", "by", "bz")):
+        # induced field against the exact B0 (the FP32 form's B0 plane is a
+        # rounded copy of the same host B0)
+        errs[n + "-b0"] = rel_l2(r32[n] - r64["b0"][i], r64[n] - r64["b0"][i])
+        errs[n] = rel_l2(r32[n], r64[n])
+    print("FP32 vs FP64 one-step relative L2:", {k: f"{v:.2e}" for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v <= 1e-5, (k, v)
+    for sp in (0, 1):
+        assert abs(r32["count"][sp] - r64["count"][sp]) <= 2
+
+
+def test_deterministic_mode_is_bit_reproducible(state):
+    f, p = state
+    a = run_one(ps.PIC_FP32, f, p, deposit=ps.PIC_DEPOSIT_DETERMINISTIC)
+    b = run_one(ps.PIC_FP32, f, p, deposit=ps.PIC_DEPOSIT_DETERMINISTIC)
+    for n in FIELDS + J:
+        assert np.array_equal(a[n], b[n]), n
+    r64 = run_one(ps.PIC_FP64, f, p)
+    for n in J:
+        assert rel_l2(a[n], r64[n]) <= 1e-5, n
+
+
+def test_multi_step_runs_with_sort_and_inflow():
+    c = cfg(1)
+    s = ps.Simulation(c)
+    s.load_particles()
+    s.begin()
+    n0 = [s.particle_count(sp) for sp in (0, 1)]
+    s.step(45)
+    d = s.diagnose()
+    for sp in (0, 1):
+        n1 = s.particle_count(sp)
+        assert abs(n1 - n0[sp]) < 0.05 * n0[sp]
+    assert np.isfinite(d["field_energy_e"]) and d["field_energy_e"] > 0
