@@ -507,6 +507,8 @@ extern "C" int pic_upload_field(pic_ctx *ctx, int which, const double *host) {
   free(tmp);
   if (e != hipSuccess)
     return set_error(ctx, PIC_E_HIP, "upload: %s", hipGetErrorString(e));
+  if ((which >= PIC_BX && which <= PIC_BZ) || (which >= PIC_B0X && which <= PIC_B0Z))
+    return f32::refresh_total_b(ctx);
   return 0;
 }
 

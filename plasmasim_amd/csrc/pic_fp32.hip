@@ -55,13 +55,15 @@ struct Push32 {
   float eimf_x, eimf_y;  // V * IMF terms added to the gathered E (:2817-2818)
   float blo[3], bhi[3];  // ballistic zone bounds in cell units
   int periodic[2];
-  double fx_scale;  // deterministic mode: fixed-point units per unit value
+  float fx;         // fixed-point units per unit value (a power of two)
+  double fx_inv;    // 1 / fx
 };
 
 // grid of planes used by the pass
 struct Grid32 {
   Geom g;
-  const float *ex, *ey, *ez, *bx, *by, *bz, *b0x, *b0y, *b0z;
+  const float *ex, *ey, *ez;
+  const float *btx, *bty, *btz;  // total B = B0 + B1, maintained for the gather
 };
 
 struct TileRun {
@@ -114,21 +116,8 @@ __device__ inline float lerp8(const float *__restrict__ F, int64_t i, int64_t dl
   return a0 * (b0 * v00 + b1 * v01) + a1 * (b0 * v10 + b1 * v11);
 }
 
-__device__ inline float lerp8s(const float *__restrict__ F,
-                               const float *__restrict__ G, int64_t i,
-                               int64_t dl, int64_t dk, float a0, float a1,
-                               float b0, float b1, float c0, float c1) {
-#define S(j) (F[i + (j)] + G[i + (j)])
-  const float v00 = c0 * S(0) + c1 * S(1);
-  const float v01 = c0 * S(dl) + c1 * S(dl + 1);
-  const float v10 = c0 * S(dk) + c1 * S(dk + 1);
-  const float v11 = c0 * S(dk + dl) + c1 * S(dk + dl + 1);
-#undef S
-  return a0 * (b0 * v00 + b1 * v01) + a1 * (b0 * v10 + b1 * v11);
-}
-
 // trilinear gather on the six staggered lattices (shape_func_v2
-// :3017-3326); B is B0 + B1
+// :3017-3326); B is the total field B0 + B1
 __device__ inline void gather32(const Grid32 &G, const AxisW32 &X,
                                 const AxisW32 &Y, const AxisW32 &Z, float f[6]) {
   const Geom &g = G.g;
@@ -136,9 +125,9 @@ __device__ inline void gather32(const Grid32 &G, const AxisW32 &X,
   f[0] = lerp8(G.ex, g.idx(X.bi, Y.bh, Z.bh), dl, dk, X.i0, X.i1, Y.h0, Y.h1, Z.h0, Z.h1);
   f[1] = lerp8(G.ey, g.idx(X.bh, Y.bi, Z.bh), dl, dk, X.h0, X.h1, Y.i0, Y.i1, Z.h0, Z.h1);
   f[2] = lerp8(G.ez, g.idx(X.bh, Y.bh, Z.bi), dl, dk, X.h0, X.h1, Y.h0, Y.h1, Z.i0, Z.i1);
-  f[3] = lerp8s(G.bx, G.b0x, g.idx(X.bh, Y.bi, Z.bi), dl, dk, X.h0, X.h1, Y.i0, Y.i1, Z.i0, Z.i1);
-  f[4] = lerp8s(G.by, G.b0y, g.idx(X.bi, Y.bh, Z.bi), dl, dk, X.i0, X.i1, Y.h0, Y.h1, Z.i0, Z.i1);
-  f[5] = lerp8s(G.bz, G.b0z, g.idx(X.bi, Y.bi, Z.bh), dl, dk, X.i0, X.i1, Y.i0, Y.i1, Z.h0, Z.h1);
+  f[3] = lerp8(G.btx, g.idx(X.bh, Y.bi, Z.bi), dl, dk, X.h0, X.h1, Y.i0, Y.i1, Z.i0, Z.i1);
+  f[4] = lerp8(G.bty, g.idx(X.bi, Y.bh, Z.bi), dl, dk, X.i0, X.i1, Y.h0, Y.h1, Z.i0, Z.i1);
+  f[5] = lerp8(G.btz, g.idx(X.bi, Y.bi, Z.bh), dl, dk, X.i0, X.i1, Y.i0, Y.i1, Z.h0, Z.h1);
 }
 
 // cal_track4 (:2804-2870) in fp32: B scaled by 1/gamma(u^{n-1/2}), gamma1
@@ -192,12 +181,14 @@ __device__ inline void drift_axis(int &k, float &o, float v, float hdt) {
 
 typedef unsigned long long u64;
 
-// fixed-point value of a contribution
-__device__ inline u64 to_fx(float v, double s) {
-  return (u64)__double2ll_rn((double)v * s);
+// Fixed-point value of one contribution.  The scale is a power of two
+// chosen so that |v * s| < 2^31 for every contribution, hence v * s is exact
+// in fp32 and rounds to an int32 that sign-extends to the 64-bit integer.
+__device__ inline u64 to_fx(float v, float s) {
+  return (u64)(long long)(int)rintf(v * s);
 }
 
-__device__ inline void lds_add(u64 *p, float v, double s) {
+__device__ inline void lds_add(u64 *p, float v, float s) {
   __hip_atomic_fetch_add((PIC_LDS u64 *)p, to_fx(v, s), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -214,11 +205,11 @@ struct GlobT<true> {
   using T = u64;
 };
 
-__device__ inline void glob_add(float *p, float v, double) {
+__device__ inline void glob_add(float *p, float v, float) {
   __hip_atomic_fetch_add((PIC_GLB float *)p, v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void glob_add(u64 *p, float v, double s) {
+__device__ inline void glob_add(u64 *p, float v, float s) {
   __hip_atomic_fetch_add((PIC_GLB u64 *)p, to_fx(v, s), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -239,39 +230,131 @@ struct Win {
   bool on;
 };
 
-// deposit of the 8 nodes of one trilinear stencil
+// Deposit of the 8 nodes of one trilinear stencil.  Lanes of a wave mostly
+// hold particles of the same cell (the particles are cell-sorted), so they
+// hit the same LDS nodes; lane j visits the nodes in the order n ^ (j & 7),
+// which spreads one instruction's lanes over different nodes and banks.
+#ifndef PIC_SWIZZLE
+#define PIC_SWIZZLE 1
+#endif
 template <typename G>
 __device__ inline void dep8(u64 *lds, G *glob, bool in_lds, const Win &w,
                             const Geom &g, int a, int b, int c, float a0,
                             float a1, float b0, float b1, float c0, float c1,
-                            float q, double s) {
+                            float q, float s) {
   const float qa0 = q * a0, qa1 = q * a1;
-  const float v[8] = {qa0 * (b0 * c0), qa0 * (b0 * c1), qa0 * (b1 * c0),
-                      qa0 * (b1 * c1), qa1 * (b0 * c0), qa1 * (b0 * c1),
-                      qa1 * (b1 * c0), qa1 * (b1 * c1)};
   if (in_lds) {
     const int WZ = w.WZ, WYZ = w.WY * w.WZ;
     const int i = ((a - w.k0) * w.WY + (b - w.l0)) * WZ + (c - w.m0);
-    lds_add(lds + i, v[0], s);
-    lds_add(lds + i + 1, v[1], s);
-    lds_add(lds + i + WZ, v[2], s);
-    lds_add(lds + i + WZ + 1, v[3], s);
-    lds_add(lds + i + WYZ, v[4], s);
-    lds_add(lds + i + WYZ + 1, v[5], s);
-    lds_add(lds + i + WYZ + WZ, v[6], s);
-    lds_add(lds + i + WYZ + WZ + 1, v[7], s);
+#if PIC_SWIZZLE
+    const int sw = threadIdx.x & 7;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const int id = n ^ sw;
+      const float wa = (id & 4) ? qa1 : qa0;
+      const float wb = (id & 2) ? b1 : b0;
+      const float wc = (id & 1) ? c1 : c0;
+      const int off = ((id & 4) ? WYZ : 0) + ((id & 2) ? WZ : 0) + (id & 1);
+      lds_add(lds + i + off, wa * (wb * wc), s);
+    }
+#else
+    lds_add(lds + i, qa0 * (b0 * c0), s);
+    lds_add(lds + i + 1, qa0 * (b0 * c1), s);
+    lds_add(lds + i + WZ, qa0 * (b1 * c0), s);
+    lds_add(lds + i + WZ + 1, qa0 * (b1 * c1), s);
+    lds_add(lds + i + WYZ, qa1 * (b0 * c0), s);
+    lds_add(lds + i + WYZ + 1, qa1 * (b0 * c1), s);
+    lds_add(lds + i + WYZ + WZ, qa1 * (b1 * c0), s);
+    lds_add(lds + i + WYZ + WZ + 1, qa1 * (b1 * c1), s);
+#endif
   } else {
     const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
     const int64_t i = g.idx(a, b, c);
-    glob_add(glob + i, v[0], s);
-    glob_add(glob + i + 1, v[1], s);
-    glob_add(glob + i + dl, v[2], s);
-    glob_add(glob + i + dl + 1, v[3], s);
-    glob_add(glob + i + dk, v[4], s);
-    glob_add(glob + i + dk + 1, v[5], s);
-    glob_add(glob + i + dk + dl, v[6], s);
-    glob_add(glob + i + dk + dl + 1, v[7], s);
+    glob_add(glob + i, qa0 * (b0 * c0), s);
+    glob_add(glob + i + 1, qa0 * (b0 * c1), s);
+    glob_add(glob + i + dl, qa0 * (b1 * c0), s);
+    glob_add(glob + i + dl + 1, qa0 * (b1 * c1), s);
+    glob_add(glob + i + dk, qa1 * (b0 * c0), s);
+    glob_add(glob + i + dk + 1, qa1 * (b0 * c1), s);
+    glob_add(glob + i + dk + dl, qa1 * (b1 * c0), s);
+    glob_add(glob + i + dk + dl + 1, qa1 * (b1 * c1), s);
   }
+}
+
+// One particle of the fused pass.  PUSH_J: hot loop A (:731-812) - ballistic
+// drift or gather + Boris, half drift, J deposit at x^{n+1/2}; DRIFT_N: hot
+// loop B (:818-867) - half drift, density deposit at x^{n+1}; BOUNDARY:
+// sr_particle (:2041-2531) on one rank.
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, typename A>
+__device__ __forceinline__ void particle_step(
+    const Grid32 &G, const Push32 &kp, const Win &w, u64 *acc, A *jx, A *jy,
+    A *jz, A *nd, int32_t &cl, float &ox, float &oy, float &oz, float &vx,
+    float &vy, float &vz) {
+  const Geom &g = G.g;
+  if (cl < 0) return;
+  int k, l, m;
+  unpack(cl, k, l, m);
+    const bool lds_ok = w.on && k >= w.vk0 && k <= w.vk1 && l >= w.vl0 &&
+                        l <= w.vl1 && m >= w.vm0 && m <= w.vm1;
+    if (PUSH_J) {
+      const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
+      const bool safe = k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 &&
+                        m >= 1 && m <= g.SZ - 2;
+      if (safe && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
+                    yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2])) {
+        float f[6];
+        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+        gather32(G, X, Y, Z, f);
+        boris32(kp, f, vx, vy, vz);
+      }
+      drift_axis(k, ox, vx, kp.hdt);
+      drift_axis(l, oy, vy, kp.hdt);
+      drift_axis(m, oz, vz, kp.hdt);
+      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
+          m <= g.SZ - 2) {
+        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+        dep8(acc + 0 * kWin, jx, lds_ok, w, g, X.bi, Y.bh, Z.bh, X.i0,
+                  X.i1, Y.h0, Y.h1, Z.h0, Z.h1, kp.qw * vx, kp.fx);
+        dep8(acc + 1 * kWin, jy, lds_ok, w, g, X.bh, Y.bi, Z.bh, X.h0,
+                  X.h1, Y.i0, Y.i1, Z.h0, Z.h1, kp.qw * vy, kp.fx);
+        dep8(acc + 2 * kWin, jz, lds_ok, w, g, X.bh, Y.bh, Z.bi, X.h0,
+                  X.h1, Y.h0, Y.h1, Z.i0, Z.i1, kp.qw * vz, kp.fx);
+      }
+    }
+    if (DRIFT_N) {
+      drift_axis(k, ox, vx, kp.hdt);
+      drift_axis(l, oy, vy, kp.hdt);
+      drift_axis(m, oz, vz, kp.hdt);
+      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
+          m <= g.SZ - 2) {
+        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+        dep8(acc + 3 * kWin, nd, lds_ok, w, g, X.bh, Y.bh, Z.bh, X.h0,
+                  X.h1, Y.h0, Y.h1, Z.h0, Z.h1, kp.w, kp.fx);
+      }
+    }
+    int32_t out = pack(k, l, m);
+    if (BOUNDARY) {
+      // sr_particle (:2041-2531): z must stay in [2, NZ+2) cells; x and y
+      // wrap periodically (shift by the box length) or drop
+      bool dead = !(m >= 2 && m < g.NZ + 2) || k < 0 || l < 0 || k >= g.SX ||
+                  l >= g.SY;
+      if (!dead) {
+        if (k >= g.NX + 2) {
+          if (kp.periodic[0]) k -= g.NX; else dead = true;
+        } else if (k < 2) {
+          if (kp.periodic[0]) k += g.NX; else dead = true;
+        }
+        if (l >= g.NY + 2) {
+          if (kp.periodic[1]) l -= g.NY; else dead = true;
+        } else if (l < 2) {
+          if (kp.periodic[1]) l += g.NY; else dead = true;
+        }
+      }
+      out = dead ? -1 : pack(k, l, m);
+    } else if (k < 0 || l < 0 || m < 0 || k >= g.SX || l >= g.SY || m >= g.SZ) {
+      out = -1;  // left the array: not representable
+    }
+    cl = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -354,90 +437,59 @@ __global__ __launch_bounds__(256) void k_particles32(
   for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin + i % WN] = 0;
   __syncthreads();
 
-  for (int64_t base = s; base < e; base += 256) {
-    const int64_t i = base + tid;
-    if (i >= e) continue;
-    const int32_t cl = P.cell[i];
-    if (cl < 0) continue;
-    int k, l, m;
-    unpack(cl, k, l, m);
-    float ox = P.ox[i], oy = P.oy[i], oz = P.oz[i];
-    float vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
-    const bool lds_ok = w.on && k >= w.vk0 && k <= w.vk1 && l >= w.vl0 &&
-                        l <= w.vl1 && m >= w.vm0 && m <= w.vm1;
-    if (PUSH_J) {
-      const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
-      const bool safe = k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 &&
-                        m >= 1 && m <= g.SZ - 2;
-      if (safe && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
-                    yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2])) {
-        float f[6];
-        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-        gather32(G, X, Y, Z, f);
-        boris32(kp, f, vx, vy, vz);
-      }
-      drift_axis(k, ox, vx, kp.hdt);
-      drift_axis(l, oy, vy, kp.hdt);
-      drift_axis(m, oz, vz, kp.hdt);
-      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
-          m <= g.SZ - 2) {
-        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-        dep8(acc + 0 * kWin, jx, lds_ok, w, g, X.bi, Y.bh, Z.bh, X.i0,
-                  X.i1, Y.h0, Y.h1, Z.h0, Z.h1, kp.qw * vx, kp.fx_scale);
-        dep8(acc + 1 * kWin, jy, lds_ok, w, g, X.bh, Y.bi, Z.bh, X.h0,
-                  X.h1, Y.i0, Y.i1, Z.h0, Z.h1, kp.qw * vy, kp.fx_scale);
-        dep8(acc + 2 * kWin, jz, lds_ok, w, g, X.bh, Y.bh, Z.bi, X.h0,
-                  X.h1, Y.h0, Y.h1, Z.i0, Z.i1, kp.qw * vz, kp.fx_scale);
-      }
+  // each thread takes 4 consecutive particles with 16-byte loads; slots of
+  // a boundary quad that belong to a neighbouring range are left untouched
+  const int64_t q0 = s >> 2, q1 = (e + 3) >> 2;
+  for (int64_t q = q0 + tid; q < q1; q += blockDim.x) {
+    const int64_t i0 = q << 2;
+    const int4 c4 = *(const int4 *)(P.cell + i0);
+    const float4 x4 = *(const float4 *)(P.ox + i0);
+    const float4 y4 = *(const float4 *)(P.oy + i0);
+    const float4 z4 = *(const float4 *)(P.oz + i0);
+    const float4 u4 = *(const float4 *)(P.vx + i0);
+    const float4 v4 = *(const float4 *)(P.vy + i0);
+    const float4 w4 = *(const float4 *)(P.vz + i0);
+    int cs[4] = {c4.x, c4.y, c4.z, c4.w};
+    float xs[4] = {x4.x, x4.y, x4.z, x4.w}, ys[4] = {y4.x, y4.y, y4.z, y4.w},
+          zs[4] = {z4.x, z4.y, z4.z, z4.w}, us[4] = {u4.x, u4.y, u4.z, u4.w},
+          vs[4] = {v4.x, v4.y, v4.z, v4.w}, ws[4] = {w4.x, w4.y, w4.z, w4.w};
+    const bool full = i0 >= s && i0 + 3 < e;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t i = i0 + j;
+      if (i < s || i >= e) continue;
+      particle_step<PUSH_J, DRIFT_N, BOUNDARY>(G, kp, w, acc, jx, jy, jz, nd,
+                                               cs[j], xs[j], ys[j], zs[j],
+                                               us[j], vs[j], ws[j]);
     }
-    if (DRIFT_N) {
-      drift_axis(k, ox, vx, kp.hdt);
-      drift_axis(l, oy, vy, kp.hdt);
-      drift_axis(m, oz, vz, kp.hdt);
-      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
-          m <= g.SZ - 2) {
-        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-        dep8(acc + 3 * kWin, nd, lds_ok, w, g, X.bh, Y.bh, Z.bh, X.h0,
-                  X.h1, Y.h0, Y.h1, Z.h0, Z.h1, kp.w, kp.fx_scale);
+    if (full) {
+      *(int4 *)(P.cell + i0) = make_int4(cs[0], cs[1], cs[2], cs[3]);
+      *(float4 *)(P.ox + i0) = make_float4(xs[0], xs[1], xs[2], xs[3]);
+      *(float4 *)(P.oy + i0) = make_float4(ys[0], ys[1], ys[2], ys[3]);
+      *(float4 *)(P.oz + i0) = make_float4(zs[0], zs[1], zs[2], zs[3]);
+      *(float4 *)(P.vx + i0) = make_float4(us[0], us[1], us[2], us[3]);
+      *(float4 *)(P.vy + i0) = make_float4(vs[0], vs[1], vs[2], vs[3]);
+      *(float4 *)(P.vz + i0) = make_float4(ws[0], ws[1], ws[2], ws[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t i = i0 + j;
+        if (i < s || i >= e) continue;
+        P.cell[i] = cs[j];
+        P.ox[i] = xs[j];
+        P.oy[i] = ys[j];
+        P.oz[i] = zs[j];
+        P.vx[i] = us[j];
+        P.vy[i] = vs[j];
+        P.vz[i] = ws[j];
       }
-    }
-    int32_t out = pack(k, l, m);
-    if (BOUNDARY) {
-      // sr_particle (:2041-2531): z must stay in [2, NZ+2) cells; x and y
-      // wrap periodically (shift by the box length) or drop
-      bool dead = !(m >= 2 && m < g.NZ + 2) || k < 0 || l < 0 || k >= g.SX ||
-                  l >= g.SY;
-      if (!dead) {
-        if (k >= g.NX + 2) {
-          if (kp.periodic[0]) k -= g.NX; else dead = true;
-        } else if (k < 2) {
-          if (kp.periodic[0]) k += g.NX; else dead = true;
-        }
-        if (l >= g.NY + 2) {
-          if (kp.periodic[1]) l -= g.NY; else dead = true;
-        } else if (l < 2) {
-          if (kp.periodic[1]) l += g.NY; else dead = true;
-        }
-      }
-      out = dead ? -1 : pack(k, l, m);
-    } else if (k < 0 || l < 0 || m < 0 || k >= g.SX || l >= g.SY || m >= g.SZ) {
-      out = -1;  // left the array: not representable
-    }
-    P.cell[i] = out;
-    if (out >= 0) {
-      P.ox[i] = ox;
-      P.oy[i] = oy;
-      P.oz[i] = oz;
-      P.vx[i] = vx;
-      P.vy[i] = vy;
-      P.vz[i] = vz;
     }
   }
   if (!w.on) return;
   __syncthreads();
   // flush the window: non-zero nodes inside the array go to the planes
   const int WYZ = w.WY * w.WZ;
-  const double inv_s = 1.0 / kp.fx_scale;
+  const double inv_s = kp.fx_inv;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     A *dst = c == 0 ? jx : c == 1 ? jy : c == 2 ? jz : nd;
@@ -480,10 +532,12 @@ __device__ inline float sigma32(const Sigma32 &s, int k2, int l2, int m2) {
   return s.amp * (0.1f * sqrtf(a4) + 0.01f * sqrtf(a12));
 }
 
-// renew_b4_2 (:2580-2611) on B1: B1 <- f B1 - h (27 D1 - D3)/24/dx
+// renew_b4_2 (:2580-2611) on B1: B1 <- f B1 - h (27 D1 - D3)/24/dx; the
+// total field B0 + B1 used by the gather is written alongside
 __global__ __launch_bounds__(256) void k_renew_b32(Geom g, Fields<float> F,
-                                                   Sigma32 sg, float dt,
-                                                   float dx, float eps0) {
+                                                   float *bt, Sigma32 sg,
+                                                   float dt, float dx,
+                                                   float eps0) {
   const int m = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   const int l = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k = 2 + blockIdx.z;
@@ -497,21 +551,28 @@ __global__ __launch_bounds__(256) void k_renew_b32(Geom g, Fields<float> F,
   s = sigma32(sg, 2 * k + 1, 2 * l, 2 * m);
   f = s == 0.0f ? 1.0f : (1.f - s * temp) / (1.f + s * temp);
   h = s == 0.0f ? dth : dth / (1.f + s * temp);
-  F.f[PIC_BX][c] = f * F.f[PIC_BX][c] -
-                   h * c24 * (27.f * (ez[c] - ez[c - dl] - ey[c] + ey[c - 1]) -
-                              (ez[c + dl] - ez[c - 2 * dl] - ey[c + 1] + ey[c - 2]));
+  const int64_t pl = g.plane;
+  float b1 = f * F.f[PIC_BX][c] -
+             h * c24 * (27.f * (ez[c] - ez[c - dl] - ey[c] + ey[c - 1]) -
+                        (ez[c + dl] - ez[c - 2 * dl] - ey[c + 1] + ey[c - 2]));
+  F.f[PIC_BX][c] = b1;
+  bt[c] = F.f[PIC_B0X][c] + b1;
   s = sigma32(sg, 2 * k, 2 * l + 1, 2 * m);
   f = s == 0.0f ? 1.0f : (1.f - s * temp) / (1.f + s * temp);
   h = s == 0.0f ? dth : dth / (1.f + s * temp);
-  F.f[PIC_BY][c] = f * F.f[PIC_BY][c] -
-                   h * c24 * (27.f * (ex[c] - ex[c - 1] - ez[c] + ez[c - dk]) -
-                              (ex[c + 1] - ex[c - 2] - ez[c + dk] + ez[c - 2 * dk]));
+  b1 = f * F.f[PIC_BY][c] -
+       h * c24 * (27.f * (ex[c] - ex[c - 1] - ez[c] + ez[c - dk]) -
+                  (ex[c + 1] - ex[c - 2] - ez[c + dk] + ez[c - 2 * dk]));
+  F.f[PIC_BY][c] = b1;
+  bt[pl + c] = F.f[PIC_B0Y][c] + b1;
   s = sigma32(sg, 2 * k, 2 * l, 2 * m + 1);
   f = s == 0.0f ? 1.0f : (1.f - s * temp) / (1.f + s * temp);
   h = s == 0.0f ? dth : dth / (1.f + s * temp);
-  F.f[PIC_BZ][c] = f * F.f[PIC_BZ][c] -
-                   h * c24 * (27.f * (ey[c] - ey[c - dk] - ex[c] + ex[c - dl]) -
-                              (ey[c + dk] - ey[c - 2 * dk] - ex[c + dl] + ex[c - 2 * dl]));
+  b1 = f * F.f[PIC_BZ][c] -
+       h * c24 * (27.f * (ey[c] - ey[c - dk] - ex[c] + ex[c - dl]) -
+                  (ey[c + dk] - ey[c - 2 * dk] - ex[c + dl] + ex[c - 2 * dl]));
+  F.f[PIC_BZ][c] = b1;
+  bt[2 * pl + c] = F.f[PIC_B0Z][c] + b1;
 }
 
 // renew_e4_2 (:2613-2652) with the curl of B1
@@ -555,13 +616,19 @@ __global__ __launch_bounds__(256) void k_renew_e32(Geom g, Fields<float> F,
 // sr_b / sr_e on one rank: the reference copies the total field into the
 // guards.  For B this form stores B1 = B - B0 and B0 is not periodic, so the
 // guard receives B1(src) + B0(src) - B0(guard); E planes pass b0 = nullptr.
-__device__ inline void copy_total(float *f, const float *b0, int64_t dst,
-                                  int64_t src) {
-  f[dst] = b0 ? f[src] + (b0[src] - b0[dst]) : f[src];
+__device__ inline void copy_total(float *f, const float *b0, float *bt,
+                                  int64_t dst, int64_t src) {
+  if (b0) {
+    f[dst] = f[src] + (b0[src] - b0[dst]);
+    bt[dst] = bt[src];
+  } else {
+    f[dst] = f[src];
+  }
 }
 
 __global__ void k_halo_x32(Geom g, float *a, float *b, float *c,
-                           const float *a0, const float *b0, const float *c0) {
+                           const float *a0, const float *b0, const float *c0,
+                           float *bt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = (int64_t)g.SY * g.SZ;
   if (i >= n) return;
@@ -570,15 +637,17 @@ __global__ void k_halo_x32(Geom g, float *a, float *b, float *c,
   const float *p0[3] = {a0, b0, c0};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    copy_total(p[q], p0[q], i, i + g.NX * dk);
-    copy_total(p[q], p0[q], i + dk, i + (g.NX + 1) * dk);
-    copy_total(p[q], p0[q], i + (g.NX + 2) * dk, i + 2 * dk);
-    copy_total(p[q], p0[q], i + (g.NX + 3) * dk, i + 3 * dk);
+    float *t = bt ? bt + q * g.plane : nullptr;
+    copy_total(p[q], p0[q], t, i, i + g.NX * dk);
+    copy_total(p[q], p0[q], t, i + dk, i + (g.NX + 1) * dk);
+    copy_total(p[q], p0[q], t, i + (g.NX + 2) * dk, i + 2 * dk);
+    copy_total(p[q], p0[q], t, i + (g.NX + 3) * dk, i + 3 * dk);
   }
 }
 
 __global__ void k_halo_y32(Geom g, float *a, float *b, float *c,
-                           const float *a0, const float *b0, const float *c0) {
+                           const float *a0, const float *b0, const float *c0,
+                           float *bt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n = (int64_t)g.SX * g.SZ;
   if (i >= n) return;
@@ -589,10 +658,11 @@ __global__ void k_halo_y32(Geom g, float *a, float *b, float *c,
   const float *p0[3] = {a0, b0, c0};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    copy_total(p[q], p0[q], o, o + g.NY * dl);
-    copy_total(p[q], p0[q], o + dl, o + (g.NY + 1) * dl);
-    copy_total(p[q], p0[q], o + (g.NY + 2) * dl, o + 2 * dl);
-    copy_total(p[q], p0[q], o + (g.NY + 3) * dl, o + 3 * dl);
+    float *t = bt ? bt + q * g.plane : nullptr;
+    copy_total(p[q], p0[q], t, o, o + g.NY * dl);
+    copy_total(p[q], p0[q], t, o + dl, o + (g.NY + 1) * dl);
+    copy_total(p[q], p0[q], t, o + (g.NY + 2) * dl, o + 2 * dl);
+    copy_total(p[q], p0[q], t, o + (g.NY + 3) * dl, o + 3 * dl);
   }
 }
 
@@ -625,6 +695,16 @@ __global__ void k_fold_y32(Geom g, Fields<float> F) {
     f[g.NY * dl] += f[0];
     f[(g.NY + 1) * dl] += f[dl];
   }
+}
+
+// total B planes from B0 + B1 over the whole array
+__global__ void k_bt_refresh(const float *__restrict__ b1,
+                             const float *__restrict__ b0, float *bt,
+                             int64_t plane, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride)
+    for (int q = 0; q < 3; ++q) bt[q * plane + i] = b0[q * plane + i] + b1[q * plane + i];
 }
 
 // deterministic mode: fixed point -> fp32 planes
@@ -941,6 +1021,7 @@ struct State32 {
   uint32_t *tile_start[2] = {nullptr, nullptr};  // per species, ntiles + 1
   int64_t sorted_end[2] = {0, 0};
   unsigned long long *fx = nullptr;  // deterministic accumulators, 8 planes
+  float *bt = nullptr;               // total B = B0 + B1, 3 planes
   bool fx_dirty = false;
   double fx_scale = 0;
 };
@@ -965,13 +1046,22 @@ static Grid32 grid32(pic_ctx *ctx) {
   G.ex = F.f[PIC_EX];
   G.ey = F.f[PIC_EY];
   G.ez = F.f[PIC_EZ];
-  G.bx = F.f[PIC_BX];
-  G.by = F.f[PIC_BY];
-  G.bz = F.f[PIC_BZ];
-  G.b0x = F.f[PIC_B0X];
-  G.b0y = F.f[PIC_B0Y];
-  G.b0z = F.f[PIC_B0Z];
+  const int64_t pl = ctx->geom.plane;
+  G.btx = st(ctx)->bt;
+  G.bty = st(ctx)->bt + pl;
+  G.btz = st(ctx)->bt + 2 * pl;
   return G;
+}
+
+int refresh_total_b(pic_ctx *ctx) {
+  Fields<float> F = fields<float>(ctx);
+  const Geom &g = ctx->geom;
+  const int64_t n = (int64_t)g.SX * g.SY * g.SZ;
+  k_bt_refresh<<<1024, kBlock, 0, ctx->stream>>>(F.f[PIC_BX], F.f[PIC_B0X],
+                                                 st(ctx)->bt, g.plane, n);
+  PIC_LAUNCH_CHECK(ctx);
+  PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
 }
 
 int init_state(pic_ctx *ctx) {
@@ -1012,7 +1102,13 @@ int init_state(pic_ctx *ctx) {
     const double qmax = c.q > 1.0 ? c.q : 1.0;
     const double unit = qmax * wmax * (c.c > 1.0 ? c.c : 1.0) /
                         (c.dx * c.dx * c.dx);
-    S->fx_scale = ldexp(1.0, 40) / unit;
+    // power of two with |contribution| * scale <= 2^30
+    S->fx_scale = ldexp(1.0, (int)floor(30.0 - log2(unit)));
+  }
+  PIC_HIP(ctx, hipMalloc(&S->bt, (size_t)3 * ctx->geom.plane * sizeof(float)));
+  {
+    int rc = refresh_total_b(ctx);
+    if (rc) return rc;
   }
   if (ctx->cfg.deposit == PIC_DEPOSIT_DETERMINISTIC) {
     PIC_HIP(ctx, hipMalloc(&S->fx, (size_t)8 * ctx->geom.plane * sizeof(unsigned long long)));
@@ -1029,6 +1125,7 @@ void free_state(pic_ctx *ctx) {
   for (int sp = 0; sp < 2; ++sp)
     if (S->tile_start[sp]) (void)hipFree(S->tile_start[sp]);
   if (S->fx) (void)hipFree(S->fx);
+  if (S->bt) (void)hipFree(S->bt);
   delete S;
   ctx->form_state = nullptr;
 }
@@ -1067,7 +1164,7 @@ int renew_b_half(pic_ctx *ctx) {
   const pic_config &c = ctx->cfg;
   dim3 b, gr = interior_grid(ctx->geom, b);
   k_renew_b32<<<gr, b, 0, ctx->stream>>>(ctx->geom, fields<float>(ctx),
-                                         sigma_f(ctx), (float)c.dt,
+                                         st(ctx)->bt, sigma_f(ctx), (float)c.dt,
                                          (float)c.dx, (float)c.eps0);
   PIC_LAUNCH_CHECK(ctx);
   return 0;
@@ -1090,10 +1187,11 @@ int halo(pic_ctx *ctx, int first) {
   const bool isb = first == PIC_BX;
   const float *a0 = isb ? F.f[PIC_B0X] : nullptr, *b0 = isb ? F.f[PIC_B0Y] : nullptr,
               *c0 = isb ? F.f[PIC_B0Z] : nullptr;
+  float *bt = isb ? st(ctx)->bt : nullptr;
   k_halo_x32<<<(unsigned)div_up((int64_t)g.SY * g.SZ, kBlock), kBlock, 0,
-               ctx->stream>>>(g, a, b, c, a0, b0, c0);
+               ctx->stream>>>(g, a, b, c, a0, b0, c0, bt);
   k_halo_y32<<<(unsigned)div_up((int64_t)g.SX * g.SZ, kBlock), kBlock, 0,
-               ctx->stream>>>(g, a, b, c, a0, b0, c0);
+               ctx->stream>>>(g, a, b, c, a0, b0, c0, bt);
   PIC_LAUNCH_CHECK(ctx);
   return 0;
 }
@@ -1155,7 +1253,9 @@ static Push32 push_const(const pic_ctx *ctx, int sp) {
   }
   p.periodic[0] = c.periodic[0];
   p.periodic[1] = c.periodic[1];
-  p.fx_scale = ctx->form_state ? ((State32 *)ctx->form_state)->fx_scale : 0.0;
+  const double sc = ctx->form_state ? ((State32 *)ctx->form_state)->fx_scale : 1.0;
+  p.fx = (float)sc;
+  p.fx_inv = 1.0 / sc;
   return p;
 }
 

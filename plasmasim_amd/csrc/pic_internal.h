@@ -219,6 +219,7 @@ int download_particles(pic_ctx *, int sp, int64_t cap, int64_t *n,
 int finish_deposit(pic_ctx *);
 int init_state(pic_ctx *);
 void free_state(pic_ctx *);
+int refresh_total_b(pic_ctx *);
 int download_raw32(pic_ctx *, int sp, int64_t cap, int64_t *n, int32_t *cell,
                    float *const *f);
 }  // namespace f32
