@@ -51,8 +51,16 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    # the C host driver (fdtd()/main_func() restated over the C-ABI)
+    drv = os.path.join(HERE, "driver")
+    cmd = ["gcc", "-O2", "-std=gnu11", "-Wall", "-o", os.path.join(drv, "pic_run"),
+           os.path.join(drv, "pic_run.c"), "-L" + HERE, "-lplasmasim",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and driver/pic_run")
     return OUT
 
 

@@ -26,13 +26,12 @@
 
 #include "philox.h"
 #include "pic_internal.h"
+#include "pic_tiles.h"
 
 namespace pic {
 namespace f32 {
 
-constexpr int kMargin = 3;      // window margin in cells (1 drift + 2 stencil)
 constexpr int kWin = 1536;      // LDS window capacity in nodes (4 x int64 each)
-constexpr int kTailChunk = 2048;
 constexpr int kPackShift = 10;
 constexpr int kPackMask = 1023;
 
@@ -66,24 +65,8 @@ struct Grid32 {
   const float *btx, *bty, *btz;  // total B = B0 + B1, maintained for the gather
 };
 
-struct TileRun {
-  const uint32_t *tile_start;  // ntiles + 1 entries
-  int ntiles;
-  int64_t sorted_end;          // [0, sorted_end) is tile-sorted
-  int64_t n;                   // stored slots
-  int ntail;                   // tail chunks of kTailChunk
-};
-
 // ---------------------------------------------------------------------------
 // small helpers
-
-__device__ inline int xcd_remap(int b, int nb) {
-  // consecutive workgroups are dealt round-robin over the 8 XCDs; give each
-  // XCD a contiguous range of tiles so neighbouring tiles share its L2
-  const int q = nb / 8;
-  if (b >= 8 * q) return b;
-  return (b % 8) * q + b / 8;
-}
 
 struct AxisW32 {
   int bi, bh;
@@ -223,13 +206,6 @@ __device__ inline void glob_flush(u64 *p, u64 v, double) {
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// window in LDS: origin (cells) and extents (nodes)
-struct Win {
-  int k0, l0, m0, WX, WY, WZ;
-  int vk0, vk1, vl0, vl1, vm0, vm1;  // cells whose stencils fit the window
-  bool on;
-};
-
 // Deposit of the 8 nodes of one trilinear stencil.  Lanes of a wave mostly
 // hold particles of the same cell (the particles are cell-sorted), so they
 // hit the same LDS nodes; lane j visits the nodes in the order n ^ (j & 7),
@@ -294,8 +270,7 @@ __device__ __forceinline__ void particle_step(
   if (cl < 0) return;
   int k, l, m;
   unpack(cl, k, l, m);
-    const bool lds_ok = w.on && k >= w.vk0 && k <= w.vk1 && l >= w.vl0 &&
-                        l <= w.vl1 && m >= w.vm0 && m <= w.vm1;
+    const bool lds_ok = in_win(w, k, l, m);
     if (PUSH_J) {
       const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
       const bool safe = k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 &&
@@ -377,53 +352,14 @@ __global__ __launch_bounds__(256) void k_particles32(
   if (tid == 0) {
     int64_t s, e;
     Win w;
-    int ak0, ak1, al0, al1, am0, am1;  // anchor box (cells)
-    if (bid < tr.ntiles) {
-      s = tr.tile_start[bid];
-      e = tr.tile_start[bid + 1];
-      const int tz = bid % tg.NTZ, ty = (bid / tg.NTZ) % tg.NTY,
-                tx = bid / (tg.NTZ * tg.NTY);
-      ak0 = 2 + tx * tg.TX;
-      ak1 = ak0 + tg.TX - 1;
-      al0 = 2 + ty * tg.TY;
-      al1 = al0 + tg.TY - 1;
-      am0 = 2 + tz * tg.TZ;
-      am1 = am0 + tg.TZ - 1;
-    } else {
-      const int c = bid - tr.ntiles;
-      s = tr.sorted_end + (int64_t)c * kTailChunk;
-      e = s + kTailChunk < tr.n ? s + kTailChunk : tr.n;
-      // anchor: bounding box of 8 sample particles of the chunk
-      ak0 = al0 = am0 = 1 << 20;
-      ak1 = al1 = am1 = -1;
-      for (int j = 0; j < 8; ++j) {
-        const int64_t i = s + (e - s - 1) * j / 7;
-        if (i < s || i >= e) continue;
-        const int32_t cl = P.cell[i];
-        if (cl < 0) continue;
-        int k, l, m;
-        unpack(cl, k, l, m);
-        ak0 = min(ak0, k); ak1 = max(ak1, k);
-        al0 = min(al0, l); al1 = max(al1, l);
-        am0 = min(am0, m); am1 = max(am1, m);
-      }
-    }
-    w.on = ak1 >= ak0;
-    if (w.on) {
-      w.k0 = ak0 - kMargin;
-      w.l0 = al0 - kMargin;
-      w.m0 = am0 - kMargin;
-      w.WX = ak1 - ak0 + 1 + 2 * kMargin;
-      w.WY = al1 - al0 + 1 + 2 * kMargin;
-      w.WZ = am1 - am0 + 1 + 2 * kMargin;
-      w.vk0 = ak0 - (kMargin - 2);
-      w.vk1 = ak1 + (kMargin - 2);
-      w.vl0 = al0 - (kMargin - 2);
-      w.vl1 = al1 + (kMargin - 2);
-      w.vm0 = am0 - (kMargin - 2);
-      w.vm1 = am1 + (kMargin - 2);
-      if ((int64_t)w.WX * w.WY * w.WZ > kWin) w.on = false;
-    }
+    tile_setup(bid, tr, tg, kWin,
+               [&](int64_t i, int &k, int &l, int &m) {
+                 const int32_t cl = P.cell[i];
+                 if (cl < 0) return false;
+                 unpack(cl, k, l, m);
+                 return true;
+               },
+               s, e, w);
     if (!(PUSH_J || DRIFT_N)) w.on = false;
     sw = w;
     s_range[0] = s;
@@ -798,24 +734,9 @@ __global__ void k_gather32(Particles32 dst, Particles32 src,
   dst.vz[i] = src.vz[j];
 }
 
-// One add of a workgroup total to a device counter: per-wave ballots and a
-// single atomic per workgroup, so that a counter shared by the whole grid
-// sees gridDim.x atomics instead of one per wave.
-__device__ inline void block_count_add(bool v, unsigned long long *counter) {
-  __shared__ unsigned int part[kBlock / 64];
-  const uint64_t ballot = __ballot(v);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (unsigned)__popcll(ballot);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int t = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
-    if (t) atomicAdd(counter, (unsigned long long)t);
-  }
-}
-
 __global__ __launch_bounds__(256) void k_sort_keys32(
     Geom g, TileGeom tg, const int32_t *__restrict__ cell, int64_t n,
-    uint32_t *keys, int64_t *valid, uint32_t *tile_count) {
+    uint32_t *keys, uint32_t *tile_count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
   const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
@@ -833,49 +754,9 @@ __global__ __launch_bounds__(256) void k_sort_keys32(
     }
     keys[i] = key;
   }
-  // per-tile counts: lanes of a wave mostly share a tile (the particles are
-  // nearly sorted), so each run of equal tiles adds once
+  // per-tile counts, one atomic per run of equal tiles in a wave
   const bool v = i < n && key < ncell;
-  const uint32_t tile = v ? key / cpt : 0xffffffffu;
-  const int lane = threadIdx.x & 63;
-  const uint32_t prev = __shfl_up(tile, 1);
-  const bool head = v && (lane == 0 || prev != tile);
-  const uint64_t heads = __ballot(head);
-  const uint64_t valid_mask = __ballot(v);
-  if (head) {
-    // the run of this head ends before the next head; count its valid lanes
-    const uint64_t after = lane == 63 ? 0 : (heads >> (lane + 1));
-    const int next = after ? lane + 1 + __ffsll((long long)after) - 1 : 64;
-    const uint64_t span = (next >= 64 ? ~0ull : ((1ull << next) - 1)) &
-                          ~((1ull << lane) - 1);
-    atomicAdd(tile_count + tile, (unsigned)__popcll(valid_mask & span));
-  }
-  block_count_add(v, (unsigned long long *)valid);
-}
-
-// exclusive scan of the per-tile counts in one workgroup: start[t], t<=nt
-__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *cnt,
-                                                     uint32_t *start, int nt) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (nt + 1023) / 1024;
-  const int lo = t * per, hi = min(nt, lo + per);
-  uint32_t s = 0;
-  for (int i = lo; i < hi; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[t] - s;
-  for (int i = lo; i < hi; ++i) {
-    start[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) start[nt] = part[1023];
+  count_tile_runs(v, v ? key / cpt : 0u, tile_count);
 }
 
 __global__ __launch_bounds__(256) void k_count_live32(
@@ -1017,9 +898,6 @@ __global__ __launch_bounds__(256) void k_diag_particles32(
 
 struct State32 {
   float *sig32 = nullptr;          // 6 x stride floats
-  uint32_t *tile_count = nullptr;  // ntiles
-  uint32_t *tile_start[2] = {nullptr, nullptr};  // per species, ntiles + 1
-  int64_t sorted_end[2] = {0, 0};
   unsigned long long *fx = nullptr;  // deterministic accumulators, 8 planes
   float *bt = nullptr;               // total B = B0 + B1, 3 planes
   bool fx_dirty = false;
@@ -1087,13 +965,6 @@ int init_state(pic_ctx *ctx) {
   free(h);
   if (e != hipSuccess)
     return set_error(ctx, PIC_E_HIP, "fp32 sigma: %s", hipGetErrorString(e));
-  const TileGeom tg = tile_geom(ctx);
-  const int nt = tg.ntiles();
-  PIC_HIP(ctx, hipMalloc(&S->tile_count, (size_t)nt * sizeof(uint32_t)));
-  for (int sp = 0; sp < 2; ++sp) {
-    PIC_HIP(ctx, hipMalloc(&S->tile_start[sp], (size_t)(nt + 1) * sizeof(uint32_t)));
-    PIC_HIP(ctx, hipMemset(S->tile_start[sp], 0, (size_t)(nt + 1) * sizeof(uint32_t)));
-  }
   {
     // fixed-point unit: 2^-40 of the largest single contribution q w c/dx^3
     // (per-node sums of a few thousand contributions stay far below 2^63)
@@ -1121,9 +992,6 @@ void free_state(pic_ctx *ctx) {
   State32 *S = st(ctx);
   if (!S) return;
   if (S->sig32) (void)hipFree(S->sig32);
-  if (S->tile_count) (void)hipFree(S->tile_count);
-  for (int sp = 0; sp < 2; ++sp)
-    if (S->tile_start[sp]) (void)hipFree(S->tile_start[sp]);
   if (S->fx) (void)hipFree(S->fx);
   if (S->bt) (void)hipFree(S->bt);
   delete S;
@@ -1223,7 +1091,7 @@ int load_particles(pic_ctx *ctx) {
           c.vdrift[2], c.seed, sp);
     PIC_LAUNCH_CHECK(ctx);
     ctx->n[sp] = n;
-    st(ctx)->sorted_end[sp] = 0;
+    ctx->sorted_end[sp] = 0;
     (void)ref_order;
   }
   // the loaded order is the reference order; bring it to tile order
@@ -1266,8 +1134,8 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
   State32 *S = st(ctx);
   const TileGeom tg = tile_geom(ctx);
   TileRun tr;
-  tr.tile_start = S->tile_start[sp];
-  tr.sorted_end = S->sorted_end[sp];
+  tr.tile_start = ctx->tile_start[sp];
+  tr.sorted_end = ctx->sorted_end[sp];
   tr.ntiles = tr.sorted_end > 0 ? tg.ntiles() : 0;
   if (tr.ntiles == 0) tr.sorted_end = 0;
   tr.n = n;
@@ -1343,7 +1211,7 @@ static int compact(pic_ctx *ctx, int sp) {
   PIC_LAUNCH_CHECK(ctx);
   ctx->cur[sp] = 1 - ctx->cur[sp];
   ctx->n[sp] = kept;
-  st(ctx)->sorted_end[sp] = 0;
+  ctx->sorted_end[sp] = 0;
   return 0;
 }
 
@@ -1386,28 +1254,28 @@ int sort(pic_ctx *ctx, int sp) {
   const int nt = tg.ntiles();
   const int64_t n = ctx->n[sp];
   if (n == 0) {
-    S->sorted_end[sp] = 0;
+    ctx->sorted_end[sp] = 0;
     return 0;
   }
   Particles32 src = p32(ctx, sp, ctx->cur[sp]);
-  PIC_HIP(ctx, hipMemsetAsync(ctx->dev_counter, 0, sizeof(int64_t), ctx->stream));
-  PIC_HIP(ctx, hipMemsetAsync(S->tile_count, 0, (size_t)nt * sizeof(uint32_t),
+  PIC_HIP(ctx, hipMemsetAsync(ctx->tile_count, 0, (size_t)nt * sizeof(uint32_t),
                               ctx->stream));
   k_sort_keys32<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
-      g, tg, src.cell, n, ctx->keys, ctx->dev_counter, S->tile_count);
+      g, tg, src.cell, n, ctx->keys, ctx->tile_count);
   PIC_LAUNCH_CHECK(ctx);
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= ncell) ++bits;
   int rc = sort_keys(ctx, ctx->keys, ctx->keys_alt, ctx->idx, n, bits);
   if (rc) return rc;
-  k_scan_tiles<<<1, 1024, 0, ctx->stream>>>(S->tile_count, S->tile_start[sp], nt);
-  PIC_LAUNCH_CHECK(ctx);
-  PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->dev_counter,
-                              sizeof(int64_t), hipMemcpyDeviceToHost,
+  rc = scan_tiles(ctx, sp);
+  if (rc) return rc;
+  // the valid count is the total of the tile counts
+  PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->tile_start[sp] + nt,
+                              sizeof(uint32_t), hipMemcpyDeviceToHost,
                               ctx->stream));
   PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int64_t valid = ctx->host_counter[0];
+  const int64_t valid = (int64_t)(*(uint32_t *)ctx->host_counter);
   Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
   if (valid > 0)
     k_gather32<<<(unsigned)div_up(valid, kBlock), kBlock, 0, ctx->stream>>>(
@@ -1415,7 +1283,7 @@ int sort(pic_ctx *ctx, int sp) {
   PIC_LAUNCH_CHECK(ctx);
   ctx->cur[sp] = 1 - ctx->cur[sp];
   ctx->n[sp] = valid;
-  S->sorted_end[sp] = valid;
+  ctx->sorted_end[sp] = valid;
   return 0;
 }
 
@@ -1548,7 +1416,7 @@ int upload_particles(pic_ctx *ctx, int sp, int64_t n, const double *const *h) {
   if (e != hipSuccess)
     return set_error(ctx, PIC_E_HIP, "upload: %s", hipGetErrorString(e));
   ctx->n[sp] = n;
-  st(ctx)->sorted_end[sp] = 0;
+  ctx->sorted_end[sp] = 0;
   return 0;
 }
 

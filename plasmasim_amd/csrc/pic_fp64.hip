@@ -13,6 +13,7 @@
 
 #include "philox.h"
 #include "pic_internal.h"
+#include "pic_tiles.h"
 
 namespace pic {
 namespace f64 {
@@ -487,6 +488,187 @@ __global__ __launch_bounds__(256) void k_particles(
   P.vz[i] = vz;
 }
 
+// ---------------------------------------------------------------------------
+// tiled particle pass: one workgroup per sort tile, J and n accumulated in an
+// LDS window with FP64 atomics, then added to the planes (see pic_tiles.h)
+
+constexpr int kWin64 = 1400;  // window capacity in nodes (4 x double each)
+
+#define PIC_LDS __attribute__((address_space(3)))
+#define PIC_GLB __attribute__((address_space(1)))
+
+__device__ inline void lds_add64(double *p, double v) {
+  __hip_atomic_fetch_add((PIC_LDS double *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void glob_add64(double *p, double v) {
+  __hip_atomic_fetch_add((PIC_GLB double *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The 8 contributions (temp*wa)*(wb*wc) of one component, as the reference
+// forms them (:3412-3425); lane j visits the nodes in the order n ^ (j & 7)
+// so that lanes holding particles of one cell hit different LDS nodes.
+__device__ inline void dep8_64(double *lds, double *glob, bool in_lds,
+                               const Win &w, const Geom &g, int a, int b,
+                               int c, double wa0, double wa1, double wb0,
+                               double wb1, double wc0, double wc1,
+                               double temp) {
+  const double ta0 = temp * wa0, ta1 = temp * wa1;
+  if (in_lds) {
+    const int WZ = w.WZ, WYZ = w.WY * w.WZ;
+    const int i = ((a - w.k0) * w.WY + (b - w.l0)) * WZ + (c - w.m0);
+    const int sw = threadIdx.x & 7;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const int id = n ^ sw;
+      const double ta = (id & 4) ? ta1 : ta0;
+      const double wb = (id & 2) ? wb1 : wb0;
+      const double wc = (id & 1) ? wc1 : wc0;
+      const int off = ((id & 4) ? WYZ : 0) + ((id & 2) ? WZ : 0) + (id & 1);
+      lds_add64(lds + i + off, (ta) * (wb * wc));
+    }
+  } else {
+    const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+    const int64_t i = g.idx(a, b, c);
+    glob_add64(glob + i, (ta0) * (wb0 * wc0));
+    glob_add64(glob + i + 1, (ta0) * (wb0 * wc1));
+    glob_add64(glob + i + dl, (ta0) * (wb1 * wc0));
+    glob_add64(glob + i + dl + 1, (ta0) * (wb1 * wc1));
+    glob_add64(glob + i + dk, (ta1) * (wb0 * wc0));
+    glob_add64(glob + i + dk + 1, (ta1) * (wb0 * wc1));
+    glob_add64(glob + i + dk + dl, (ta1) * (wb1 * wc0));
+    glob_add64(glob + i + dk + dl + 1, (ta1) * (wb1 * wc1));
+  }
+}
+
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY>
+__global__ __launch_bounds__(256) void k_particles_tiled(
+    Geom g, TileGeom tg, Fields<double> F, Particles64 P, TileRun tr,
+    StepConst kc, double *jx, double *jy, double *jz, double *nd) {
+  __shared__ double acc[4 * kWin64];
+  __shared__ Win sw;
+  __shared__ int64_t s_range[2];
+  const int tid = threadIdx.x;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const double dx = kc.dx;
+  if (tid == 0) {
+    int64_t s, e;
+    Win w;
+    tile_setup(bid, tr, tg, kWin64,
+               [&](int64_t i, int &k, int &l, int &m) {
+                 const double x = P.x[i], y = P.y[i], z = P.z[i];
+                 if (!in_array(g, x, y, z, dx)) return false;
+                 k = (int)(x / dx);
+                 l = (int)(y / dx);
+                 m = (int)(z / dx);
+                 return true;
+               },
+               s, e, w);
+    if (!(PUSH_J || DRIFT_N)) w.on = false;
+    sw = w;
+    s_range[0] = s;
+    s_range[1] = e;
+  }
+  __syncthreads();
+  const int64_t s = s_range[0], e = s_range[1];
+  if (s >= e) return;
+  const Win w = sw;
+  const int WN = w.on ? w.WX * w.WY * w.WZ : 0;
+  for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin64 + i % WN] = 0.0;
+  __syncthreads();
+  const double dth = kc.dt * 0.5;
+  for (int64_t i = s + tid; i < e; i += blockDim.x) {
+    double z = P.z[i];
+    if (isnan(z)) continue;
+    double x = P.x[i], y = P.y[i];
+    double vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+    const bool safe0 = in_array(g, x, y, z, dx);
+    const bool lds_ok =
+        safe0 && in_win(w, (int)(x / dx), (int)(y / dx), (int)(z / dx));
+    if (PUSH_J) {
+      if (x < kc.blo[0] || x > kc.bhi[0] || y < kc.blo[1] || y > kc.bhi[1] ||
+          z < kc.blo[2] || z > kc.bhi[2] || !safe0) {
+        x += vx * dth;
+        y += vy * dth;
+        z += vz * dth;
+      } else {
+        double f[6];
+        gather(F, g, x, y, z, dx, f);
+        boris(kc, f, x, y, z, vx, vy, vz);
+      }
+      if (in_gate(g, x, y, z, dx) && in_array(g, x, y, z, dx)) {
+        // shape_func_*0_j_2 (:3397-3553) with the reference's weights
+        const double qn = kc.q * kc.weight / (dx * dx * dx);
+        const AxisW X = axis_weights(x, dx), Y = axis_weights(y, dx),
+                    Z = axis_weights(z, dx);
+        dep8_64(acc + 0 * kWin64, jx, lds_ok, w, g, X.bi, Y.bh, Z.bh, X.iu,
+                X.il, Y.hu, Y.hl, Z.hu, Z.hl, qn * vx);
+        dep8_64(acc + 1 * kWin64, jy, lds_ok, w, g, X.bh, Y.bi, Z.bh, X.hu,
+                X.hl, Y.iu, Y.il, Z.hu, Z.hl, qn * vy);
+        dep8_64(acc + 2 * kWin64, jz, lds_ok, w, g, X.bh, Y.bh, Z.bi, X.hu,
+                X.hl, Y.hu, Y.hl, Z.iu, Z.il, qn * vz);
+      }
+    }
+    if (DRIFT_N) {
+      x += vx * dth;
+      y += vy * dth;
+      z += vz * dth;
+      if (in_gate(g, x, y, z, dx) && in_array(g, x, y, z, dx)) {
+        // shape_func_*0_n_2 (:3333-3395)
+        const AxisW X = axis_weights(x, dx), Y = axis_weights(y, dx),
+                    Z = axis_weights(z, dx);
+        dep8_64(acc + 3 * kWin64, nd, lds_ok, w, g, X.bh, Y.bh, Z.bh, X.hu,
+                X.hl, Y.hu, Y.hl, Z.hu, Z.hl, kc.weight / (dx * dx * dx));
+      }
+    }
+    if (BOUNDARY) {
+      const double dx2 = 2. * dx;
+      const double gx2 = (g.NX + 2) * dx, gy2 = (g.NY + 2) * dx,
+                   gz2 = (g.NZ + 2) * dx;
+      bool dead = !(z < gz2 && z >= dx2);
+      if (!dead) {
+        if (x >= gx2) {
+          if (kc.periodic[0]) x = x - g.NX * dx; else dead = true;
+        } else if (x < dx2) {
+          if (kc.periodic[0]) x = x + g.NX * dx; else dead = true;
+        }
+        if (y >= gy2) {
+          if (kc.periodic[1]) y = y - g.NY * dx; else dead = true;
+        } else if (y < dx2) {
+          if (kc.periodic[1]) y = y + g.NY * dx; else dead = true;
+        }
+      }
+      if (dead) {
+        P.z[i] = __longlong_as_double(0x7ff8000000000000LL);
+        continue;
+      }
+    }
+    P.x[i] = x;
+    P.y[i] = y;
+    P.z[i] = z;
+    P.vx[i] = vx;
+    P.vy[i] = vy;
+    P.vz[i] = vz;
+  }
+  if (!w.on) return;
+  __syncthreads();
+  const int WYZ = w.WY * w.WZ;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double *dst = c == 0 ? jx : c == 1 ? jy : c == 2 ? jz : nd;
+    for (int j = tid; j < WN; j += blockDim.x) {
+      const double v = acc[c * kWin64 + j];
+      if (v == 0.0) continue;
+      const int kk = w.k0 + j / WYZ, ll = w.l0 + (j / w.WZ) % w.WY,
+                mm = w.m0 + j % w.WZ;
+      if (kk < 0 || ll < 0 || mm < 0 || kk >= g.SX || ll >= g.SY || mm >= g.SZ)
+        continue;
+      glob_add64(dst + g.idx(kk, ll, mm), v);
+    }
+  }
+}
+
 // init_particle_flat (:412-625): particle p sits in interior cell p/ppc,
 // cells in (k,l,m) order with m fastest; Philox event 0.
 __global__ void k_load(Geom g, Particles64 P, int64_t n, int ppc, double dx,
@@ -562,36 +744,49 @@ __global__ void k_gather_particles(Particles64 dst, Particles64 src,
 }
 
 // sorting key (:7488-7497): ((int)(x/dx)-2, (int)(y/dx)-2, (int)(z/dx)-2),
-// lexicographic with m fastest; particles outside the interior (and dead
-// ones) get the key ncell and are dropped after the sort.
-__global__ __launch_bounds__(256) void k_sort_keys(Geom g, TileGeom tg, Particles64 P, int64_t n,
-                            double dx, uint32_t *keys, int64_t *valid) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// tile-major (the reference's lexicographic order with m fastest when one
+// tile spans the interior); particles outside the interior (and dead ones)
+// get the key ncell and are dropped after the sort.  Grid-stride loop with a
+// bounded grid so the valid counter sees one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_sort_keys(Geom g, TileGeom tg,
+                                                   Particles64 P, int64_t n,
+                                                   double dx, uint32_t *keys,
+                                                   int64_t *valid,
+                                                   uint32_t *tile_count) {
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
-  uint32_t key = ncell;
-  if (i < n) {
-    const double x = P.x[i], y = P.y[i], z = P.z[i];
-    const double xd = x / dx, yd = y / dx, zd = z / dx;
-    // range test before the conversions so that (int) never sees a value
-    // outside int range or NaN
-    if (xd > -1.0 && xd < g.NX + 3 && yd > -1.0 && yd < g.NY + 3 &&
-        zd > -1.0 && zd < g.NZ + 3) {
-      const int k = (int)(xd) - 2, l = (int)(yd) - 2, m = (int)(zd) - 2;
-      if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
-        key = tg.key(k, l, m);
+  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
+  unsigned int cnt = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nround = (n + stride - 1) / stride * stride;
+  // whole waves iterate together and reach the per-wave tile counting from
+  // one call site, so its ballots see every lane
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nround;
+       i += stride) {
+    uint32_t key = ncell;
+    if (i < n) {
+      const double x = P.x[i], y = P.y[i], z = P.z[i];
+      const double xd = x / dx, yd = y / dx, zd = z / dx;
+      // range test before the conversions so that (int) never sees a value
+      // outside int range or NaN
+      if (xd > -1.0 && xd < g.NX + 3 && yd > -1.0 && yd < g.NY + 3 &&
+          zd > -1.0 && zd < g.NZ + 3) {
+        const int k = (int)(xd) - 2, l = (int)(yd) - 2, m = (int)(zd) - 2;
+        if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
+          key = tg.key(k, l, m);
+      }
+      keys[i] = key;
+      cnt += key < ncell;
     }
-    keys[i] = key;
+    count_tile_runs(key < ncell, key < ncell ? key / cpt : 0u, tile_count);
   }
-  // count valid keys: per-wave ballots, one atomic per workgroup
-  const bool v = i < n && key < ncell;
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
   __shared__ unsigned int part[kBlock / 64];
-  const uint64_t ballot = __ballot(v);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (unsigned)__popcll(ballot);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned int t = 0;
+    unsigned long long t = 0;
     for (int w = 0; w < kBlock / 64; ++w) t += part[w];
-    if (t) atomicAdd((unsigned long long *)valid, (unsigned long long)t);
+    if (t) atomicAdd((unsigned long long *)valid, t);
   }
 }
 
@@ -805,8 +1000,15 @@ int load_particles(pic_ctx *ctx) {
           ctx->geom, P, n, c.ppc, c.dx, c.vth[sp], c.vdrift[0], c.vdrift[1],
           c.vdrift[2], c.seed, sp);
     ctx->n[sp] = n;
+    ctx->sorted_end[sp] = 0;
   }
   PIC_LAUNCH_CHECK(ctx);
+  // the loaded order is the reference's; bring it to tile order (the same
+  // order when one tile spans the interior)
+  for (int sp = 0; sp < 2; ++sp) {
+    int rc = sort(ctx, sp);
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -821,11 +1023,20 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
   double *jy = F.f[sp == PIC_ION ? PIC_JIY : PIC_JEY];
   double *jz = F.f[sp == PIC_ION ? PIC_JIZ : PIC_JEZ];
   double *nd = F.f[sp == PIC_ION ? PIC_NI : PIC_NE];
-  const unsigned nb = (unsigned)div_up(n, kBlock);
+  const TileGeom tg = tile_geom(ctx);
+  TileRun tr;
+  tr.tile_start = ctx->tile_start[sp];
+  tr.sorted_end = ctx->sorted_end[sp];
+  // a single tile (the reference's order) is processed as tail chunks
+  tr.ntiles = (tr.sorted_end > 0 && tg.ntiles() > 1) ? tg.ntiles() : 0;
+  if (tr.ntiles == 0) tr.sorted_end = 0;
+  tr.n = n;
+  tr.ntail = (int)div_up(n - tr.sorted_end, kTailChunk);
+  const unsigned nb = (unsigned)(tr.ntiles + tr.ntail);
   const int sel = (do_push_j ? 4 : 0) | (do_drift_n ? 2 : 0) | (do_boundary ? 1 : 0);
-#define LAUNCH(A, B, C)                                                   \
-  k_particles<A, B, C><<<nb, kBlock, 0, ctx->stream>>>(ctx->geom, F, P, n, \
-                                                       kc, jx, jy, jz, nd)
+#define LAUNCH(A, B, C)                                                    \
+  k_particles_tiled<A, B, C><<<nb, kBlock, 0, ctx->stream>>>(               \
+      ctx->geom, tg, F, P, tr, kc, jx, jy, jz, nd)
   switch (sel) {
     case 7: LAUNCH(true, true, true); break;
     case 6: LAUNCH(true, true, false); break;
@@ -860,21 +1071,37 @@ static int compact(pic_ctx *ctx, int sp, int64_t *kept) {
   return 0;
 }
 
+// add_*_flat: slab particles become dead slots (z = NaN) in place and the
+// refill is appended, so tile ranges stay valid until the next sort; the
+// store is compacted first only when the refill would not fit
+__global__ void k_inflow_kill(double *z, int64_t n, double lo, double hi) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = z[i];
+  if (!isnan(v) && (v < lo || v > hi)) z[i] = __longlong_as_double(0x7ff8000000000000LL);
+}
+
 int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   const pic_config &c = ctx->cfg;
   const Geom &g = ctx->geom;
-  const int64_t n = ctx->n[sp];
-  if (n > 0) {
+  if (ctx->n[sp] > 0) {
     Particles64 P = p64(ctx, sp, ctx->cur[sp]);
-    k_inflow_flags<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
-        P.z, n, 4 * c.dx, g.NZ * c.dx, ctx->flags);
+    k_inflow_kill<<<(unsigned)div_up(ctx->n[sp], kBlock), kBlock, 0,
+                    ctx->stream>>>(P.z, ctx->n[sp], 4 * c.dx, g.NZ * c.dx);
+    PIC_LAUNCH_CHECK(ctx);
+  }
+  const int nface = c.refill_far ? 2 : 1;
+  const int64_t add = (int64_t)g.NX * g.NY * 2 * c.ppc * nface;
+  if (ctx->n[sp] + add > ctx->cap) {
+    Particles64 P = p64(ctx, sp, ctx->cur[sp]);
+    k_live_flags<<<(unsigned)div_up(ctx->n[sp], kBlock), kBlock, 0,
+                   ctx->stream>>>(P.z, ctx->n[sp], ctx->flags);
     PIC_LAUNCH_CHECK(ctx);
     int64_t kept = 0;
     int rc = compact(ctx, sp, &kept);
     if (rc) return rc;
+    ctx->sorted_end[sp] = 0;
   }
-  const int nface = c.refill_far ? 2 : 1;
-  const int64_t add = (int64_t)g.NX * g.NY * 2 * c.ppc * nface;
   if (ctx->n[sp] + add > ctx->cap)
     return set_error(ctx, PIC_E_CAPACITY,
                      "inflow: %lld + %lld particles exceed capacity %lld",
@@ -895,14 +1122,21 @@ int sort(pic_ctx *ctx, int sp) {
   const int64_t n = ctx->n[sp];
   if (n == 0) return 0;
   Particles64 src = p64(ctx, sp, ctx->cur[sp]);
+  const TileGeom tg = tile_geom(ctx);
+  const int nt = tg.ntiles();
   PIC_HIP(ctx, hipMemsetAsync(ctx->dev_counter, 0, sizeof(int64_t), ctx->stream));
-  k_sort_keys<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
-      g, tile_geom(ctx), src, n, ctx->cfg.dx, ctx->keys, ctx->dev_counter);
+  PIC_HIP(ctx, hipMemsetAsync(ctx->tile_count, 0, (size_t)nt * sizeof(uint32_t),
+                              ctx->stream));
+  k_sort_keys<<<(unsigned)(div_up(n, kBlock) < 4096 ? div_up(n, kBlock) : 4096),
+                kBlock, 0, ctx->stream>>>(g, tg, src, n, ctx->cfg.dx, ctx->keys,
+                                          ctx->dev_counter, ctx->tile_count);
   PIC_LAUNCH_CHECK(ctx);
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= ncell) ++bits;
   int rc = sort_keys(ctx, ctx->keys, ctx->keys_alt, ctx->idx, n, bits);
+  if (rc) return rc;
+  rc = scan_tiles(ctx, sp);
   if (rc) return rc;
   PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->dev_counter,
                               sizeof(int64_t), hipMemcpyDeviceToHost,
@@ -916,6 +1150,7 @@ int sort(pic_ctx *ctx, int sp) {
   PIC_LAUNCH_CHECK(ctx);
   ctx->cur[sp] = 1 - ctx->cur[sp];
   ctx->n[sp] = valid;
+  ctx->sorted_end[sp] = valid;
   return 0;
 }
 

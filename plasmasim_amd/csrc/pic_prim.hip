@@ -57,4 +57,37 @@ int sort_keys(pic_ctx *ctx, uint32_t *keys_in, uint32_t *keys_out,
   return 0;
 }
 
+// exclusive scan of the per-tile counts in one workgroup: start[t], t<=nt
+__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *cnt,
+                                                     uint32_t *start, int nt) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (nt + 1023) / 1024;
+  const int lo = t * per, hi = min(nt, lo + per);
+  uint32_t s = 0;
+  for (int i = lo; i < hi; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;
+  for (int i = lo; i < hi; ++i) {
+    start[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) start[nt] = part[1023];
+}
+
+int scan_tiles(pic_ctx *ctx, int sp) {
+  const int nt = tile_geom(ctx).ntiles();
+  k_scan_tiles<<<1, 1024, 0, ctx->stream>>>(ctx->tile_count,
+                                            ctx->tile_start[sp], nt);
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
 }  // namespace pic

@@ -1,0 +1,125 @@
+/*
+ * pic_run.c - host driver of the 3D EM-PIC step, in C over the C-ABI.
+ *
+ * Restates the reference's main_func()/fdtd() (ashida-lab/PLASMAsim
+ * PIC/3d_pic458_domain.c :256-289, :627-952) on one subdomain: set up the
+ * grid and the static dipole (init_grid), load the plasma
+ * (init_particle_flat), the pre-loop inflow (:673-674), then the step loop
+ * calling one library routine per reference routine in the reference's
+ * order, with the sort and inflow cadences and a periodic diagnostics line
+ * (thrust_f, every 500 steps in the reference; here every --diag steps).
+ *
+ * Usage: pic_run [--preset N] [--steps K] [--fp64|--fp32] [--det]
+ *                [--diag D] [--fused]
+ * --fused calls pic_step() (loops A and B fused into one particle pass)
+ * instead of the routine-by-routine loop.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/plasmasim.h"
+
+#define CHECK(ctx, call)                                                   \
+  do {                                                                     \
+    int rc_ = (call);                                                      \
+    if (rc_) {                                                             \
+      fprintf(stderr, "%s failed (%d): %s\n", #call, rc_,                  \
+              (ctx) ? pic_last_error(ctx) : "");                           \
+      if (ctx) pic_destroy(ctx);                                           \
+      return 1;                                                            \
+    }                                                                      \
+  } while (0)
+
+static double now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* one reference step: fdtd() body :687-949 */
+static int step_routines(pic_ctx *ctx, const pic_config *c, int *c_sort,
+                         int *c_add) {
+  int rc;
+  if ((rc = pic_renew_grid(ctx))) return rc;                    /* :688 */
+  if (*c_sort > c->sort_every - 1) {                            /* :694 */
+    *c_sort = 0;
+    if ((rc = pic_sort(ctx, PIC_ION))) return rc;
+    if ((rc = pic_sort(ctx, PIC_ELECTRON))) return rc;
+  }
+  if (*c_add > c->inflow_every - 1) {                           /* :700 */
+    if ((rc = pic_inflow(ctx))) return rc;
+    *c_add = 0;
+  }
+  if ((rc = pic_renew_b_half(ctx))) return rc;                  /* :727 */
+  if ((rc = pic_halo_b(ctx))) return rc;                        /* :729 */
+  if ((rc = pic_push_deposit_j(ctx, PIC_ION))) return rc;       /* :731-812 */
+  if ((rc = pic_push_deposit_j(ctx, PIC_ELECTRON))) return rc;
+  if ((rc = pic_renew_b_half(ctx))) return rc;                  /* :814 */
+  if ((rc = pic_halo_b(ctx))) return rc;                        /* :816 */
+  if ((rc = pic_drift_deposit_n(ctx, PIC_ION))) return rc;      /* :818-867 */
+  if ((rc = pic_drift_deposit_n(ctx, PIC_ELECTRON))) return rc;
+  if ((rc = pic_fold_current(ctx))) return rc;                  /* :893 */
+  if ((rc = pic_renew_e(ctx))) return rc;                       /* :895 */
+  if ((rc = pic_halo_e(ctx))) return rc;                        /* :899 */
+  if ((rc = pic_particle_boundary(ctx))) return rc;             /* :909-911 */
+  (*c_sort)++;
+  (*c_add)++;
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  int preset = 1, steps = 100, diag = 50, fused = 0;
+  int precision = -1, det = 0;
+  for (int i = 1; i < argc; ++i) {
+    if (!strcmp(argv[i], "--preset") && i + 1 < argc) preset = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--steps") && i + 1 < argc) steps = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--diag") && i + 1 < argc) diag = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--fp64")) precision = PIC_FP64;
+    else if (!strcmp(argv[i], "--fp32")) precision = PIC_FP32;
+    else if (!strcmp(argv[i], "--det")) det = 1;
+    else if (!strcmp(argv[i], "--fused")) fused = 1;
+    else {
+      fprintf(stderr,
+              "usage: %s [--preset N] [--steps K] [--fp64|--fp32] [--det] "
+              "[--diag D] [--fused]\n", argv[0]);
+      return 2;
+    }
+  }
+  pic_config cfg;
+  pic_ctx *ctx = NULL;
+  CHECK(ctx, pic_config_preset(preset, &cfg));
+  if (precision >= 0) cfg.precision = precision;
+  if (det) cfg.deposit = PIC_DEPOSIT_DETERMINISTIC;
+  CHECK(ctx, pic_create(&cfg, &ctx));                 /* init_grid :291 */
+  CHECK(ctx, pic_load_particles(ctx));                /* :412-625 */
+  CHECK(ctx, pic_begin(ctx));                         /* :673-674 */
+  fprintf(stderr, "grid %dx%dx%d ppc %d form %s deposit %s\n", cfg.nx, cfg.ny,
+          cfg.nz, cfg.ppc, cfg.precision == PIC_FP64 ? "fp64" : "fp32",
+          cfg.deposit == PIC_DEPOSIT_DETERMINISTIC ? "deterministic" : "atomic");
+  int c_sort = 0, c_add = 0;
+  const double t0 = now();
+  for (int c = 0; c < steps; ++c) {
+    if (fused) {
+      CHECK(ctx, pic_step(ctx, 1));
+    } else {
+      CHECK(ctx, step_routines(ctx, &cfg, &c_sort, &c_add));
+    }
+    if (diag > 0 && ((c + 1) % diag == 0 || c + 1 == steps)) {
+      pic_diag d;
+      CHECK(ctx, pic_diagnose(ctx, &d));
+      const double W = d.field_energy_e + d.field_energy_b1 + d.kinetic[0] +
+                       d.kinetic[1];
+      printf("%d %.3f s  Ee %.6e  Eb1 %.6e  Ki %.6e  Ke %.6e  total %.6e  "
+             "Fx %.4e Fy %.4e Fz %.4e  gauss_l2 %.4e  n %lld %lld\n",
+             c + 1, now() - t0, d.field_energy_e, d.field_energy_b1,
+             d.kinetic[0], d.kinetic[1], W, d.force[0], d.force[1],
+             d.force[2], d.gauss_l2, (long long)d.count[0],
+             (long long)d.count[1]);
+      fflush(stdout);
+    }
+  }
+  pic_destroy(ctx);
+  return 0;
+}
