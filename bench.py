@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # algorithmic bytes per particle-step of the fused pass (read + write state)
 BYTES_PER_PARTICLE = {"fp64": 96, "fp32": 56}
+# HBM bytes per particle-step of the same pass measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE at this workload, scaled by the counts of a
+# 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md)
+TRAFFIC_PER_PARTICLE = {"fp64": 103.4, "fp32": 62.2}
 
 
 def run_form(ps, form, steps, warmup, nz=None):
@@ -34,6 +38,11 @@ def run_form(ps, form, steps, warmup, nz=None):
     sim = ps.Simulation(c)
     sim.load_particles()
     sim.begin()
+    # live per-kernel timing over every step of the run (HIP events on the
+    # library's stream, a few microseconds per step), so the kernel average
+    # covers the same launches a kernel trace of this command does
+    sim.timing(True)
+    sim.timing_reset()
     sim.step(warmup)
     sim.sync()
     n0 = sim.particle_count(0) + sim.particle_count(1)
@@ -44,13 +53,7 @@ def run_form(ps, form, steps, warmup, nz=None):
     n1 = sim.particle_count(0) + sim.particle_count(1)
     particles = 0.5 * (n0 + n1)
     rate = particles * steps / (t1 - t0)
-    # live per-kernel timing of the same step loop (HIP events on the
-    # library's stream), in a separate pass so the timed region above has no
-    # extra event records
-    sim.timing(True)
-    sim.timing_reset()
-    tsteps = max(5, min(steps, 20))
-    sim.step(tsteps)
+    tsteps = warmup + steps
     push_ms, push_n = sim.timing_get("push")
     tot = {k: sim.timing_get(k) for k in ("push", "field_b", "field_e", "sort", "inflow", "other")}
     sim.timing(False)
@@ -62,6 +65,7 @@ def run_form(ps, form, steps, warmup, nz=None):
         "rate": rate, "ms_per_step": (t1 - t0) / steps * 1e3,
         "particles": particles, "push_avg_ms": push_avg_s * 1e3,
         "achieved": achieved, "kernel_ms": {k: v[0] / tsteps for k, v in tot.items()},
+        "traffic": TRAFFIC_PER_PARTICLE[form] * per_launch_particles,
     }
 
 
@@ -93,6 +97,22 @@ def cpu_baseline(ps):
                       f"({n} particles), {steps} steps"}
 
 
+def calibrate_copy(mib):
+    """One device-to-device copy of `mib` MiB: reads and writes exactly
+    mib MiB, so the HBM byte counters of its dispatch give the factor that
+    turns counter readings into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {mib} MiB read, {mib} MiB written", file=sys.stderr)
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,12 +121,17 @@ def main():
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a plain "
+                         "copy kernel of known size, to scale HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(one subdomain per GPU; see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+    if a.calibrate_copy > 0:
+        calibrate_copy(a.calibrate_copy)
     import plasmasim_amd as ps
     main_r = run_form(ps, a.form, a.steps, a.warmup)
     other = None
@@ -132,7 +157,8 @@ def main():
                    "form": a.form},
         "roofline": {"bound": "hbm", "achieved": main_r["achieved"] / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": main_r["achieved"] / HBM_PEAK, "traffic": None,
+                     "frac": main_r["achieved"] / HBM_PEAK,
+                     "traffic": main_r["traffic"],
                      "kernel": "fused particle pass (gather, push, J and n deposit, boundary)",
                      "kernel_avg_ms": main_r["push_avg_ms"],
                      "bytes_per_particle": BYTES_PER_PARTICLE[a.form]},
