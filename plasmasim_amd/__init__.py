@@ -190,3 +190,137 @@ class Simulation:
         self._check(self._L.pic_timing_get(self._ctx, _lib.TIMING[which],
                                            ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+# ---------------------------------------------------------------------------
+# 2D3V variant (include/plasmasim2d.h; reference PIC2D/2d_pic278_mpi.c)
+
+from ._lib import FIELDS2D, FIELD2D_ID, Pic2dConfig, Pic2dDiag  # noqa: E402
+
+
+def preset2d(n):
+    """2D workload preset n (0 = 32 x 64 parity case, 1 = 256^2, 3 = 2048^2)."""
+    c = Pic2dConfig()
+    rc = lib().pic2d_config_preset(int(n), ctypes.byref(c))
+    if rc:
+        raise PicError(rc, f"unknown 2D preset {n}")
+    return c
+
+
+class Simulation2D:
+    """One pic2d_ctx: one method per routine of the 2D fdtd() (:527-1019)."""
+
+    _ROUTINES = ("load_particles", "begin", "renew_grid", "inflow",
+                 "renew_b_half", "halo", "fold_current", "fold_density",
+                 "renew_e", "poisson", "particle_boundary", "sync")
+
+    def __init__(self, cfg):
+        self._L = lib()
+        self.cfg = cfg.copy()
+        ptr = ctypes.c_void_p()
+        rc = self._L.pic2d_create(ctypes.byref(self.cfg), ctypes.byref(ptr))
+        if rc:
+            raise PicError(rc, "pic2d_create failed (see stderr)")
+        self._ctx = ptr
+        self.shape = (cfg.nx + 4, cfg.nz + 4)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.pic2d_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            msg = self._L.pic2d_last_error(self._ctx)
+            raise PicError(rc, msg.decode() if msg else "")
+
+    def __getattr__(self, name):
+        if name in Simulation2D._ROUTINES:
+            fn = getattr(self._L, "pic2d_" + name)
+            return lambda: self._check(fn(self._ctx))
+        raise AttributeError(name)
+
+    def upload_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        self._check(self._L.pic2d_upload_field(self._ctx, FIELD2D_ID[name], _lib.dptr(a)))
+
+    def download_field(self, name):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._check(self._L.pic2d_download_field(self._ctx, FIELD2D_ID[name], _lib.dptr(a)))
+        return a
+
+    def upload_particles(self, sp, parts):
+        """parts: (n, 5) array of x, z, vx, vy, vz."""
+        p = np.asarray(parts, dtype=np.float64)
+        cols = [np.ascontiguousarray(p[:, j]) for j in range(5)]
+        self._check(self._L.pic2d_upload_particles(
+            self._ctx, sp, len(p), *[_lib.dptr(c) for c in cols]))
+
+    def particle_count(self, sp):
+        n = ctypes.c_int64()
+        self._check(self._L.pic2d_particle_count(self._ctx, sp, ctypes.byref(n)))
+        return n.value
+
+    def download_particles(self, sp):
+        cap = max(self.particle_count(sp), 1)
+        n = ctypes.c_int64()
+        cols = [np.empty(cap) for _ in range(5)]
+        self._check(self._L.pic2d_download_particles(
+            self._ctx, sp, cap, ctypes.byref(n), *[_lib.dptr(c) for c in cols]))
+        return np.stack([c[:n.value] for c in cols], axis=1)
+
+    def push_deposit_j(self, sp):
+        self._check(self._L.pic2d_push_deposit_j(self._ctx, sp))
+
+    def drift_deposit_n(self, sp):
+        self._check(self._L.pic2d_drift_deposit_n(self._ctx, sp))
+
+    def external_current(self, step):
+        self._check(self._L.pic2d_external_current(self._ctx, int(step)))
+
+    def sort(self, sp):
+        self._check(self._L.pic2d_sort(self._ctx, sp))
+
+    def step(self, n=1):
+        self._check(self._L.pic2d_step(self._ctx, int(n)))
+
+    def step_index(self):
+        s = ctypes.c_int64()
+        self._check(self._L.pic2d_step_index(self._ctx, ctypes.byref(s)))
+        return s.value
+
+    def diagnose(self):
+        d = Pic2dDiag()
+        self._check(self._L.pic2d_diagnose(self._ctx, ctypes.byref(d)))
+        return d.to_dict()
+
+    def probe(self, k, m):
+        out = np.empty(8)
+        self._check(self._L.pic2d_probe(self._ctx, int(k), int(m), _lib.dptr(out)))
+        return out
+
+    def timing(self, on=True):
+        self._check(self._L.pic2d_timing_enable(self._ctx, 1 if on else 0))
+
+    def timing_reset(self):
+        self._check(self._L.pic2d_timing_reset(self._ctx))
+
+    def timing_get(self, which):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self._L.pic2d_timing_get(self._ctx, _lib.TIMING[which],
+                                             ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

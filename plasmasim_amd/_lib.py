@@ -76,6 +76,52 @@ class PicDiag(ctypes.Structure):
         return d
 
 
+class Pic2dConfig(ctypes.Structure):
+    """pic2d_config of include/plasmasim2d.h."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("nz", ctypes.c_int32), ("ppc", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dt", ctypes.c_double),
+        ("c", ctypes.c_double), ("eps0", ctypes.c_double),
+        ("mu0", ctypes.c_double), ("q", ctypes.c_double),
+        ("mass", ctypes.c_double * 2), ("weight", ctypes.c_double * 2),
+        ("vth", ctypes.c_double * 2), ("vdrift", ctypes.c_double),
+        ("imf", ctypes.c_double * 3),
+        ("ext_current", ctypes.c_double), ("ext_ramp", ctypes.c_int32),
+        ("ext_k", ctypes.c_int32), ("ext_m", ctypes.c_int32 * 2),
+        ("absorb", ctypes.c_int32 * 2), ("sigma_amp", ctypes.c_double),
+        ("ballistic_lo", ctypes.c_double * 2),
+        ("ballistic_hi", ctypes.c_double * 2),
+        ("phi_lo", ctypes.c_int32 * 2), ("phi_hi", ctypes.c_int32 * 2),
+        ("poisson_every", ctypes.c_int32), ("inflow_every", ctypes.c_int32),
+        ("sort_every", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("deposit", ctypes.c_int32), ("tile", ctypes.c_int32 * 2),
+        ("seed", ctypes.c_uint64), ("capacity", ctypes.c_int64),
+    ]
+
+    def copy(self):
+        c = Pic2dConfig()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(self), ctypes.sizeof(self))
+        return c
+
+
+class Pic2dDiag(ctypes.Structure):
+    """pic2d_diag of include/plasmasim2d.h."""
+    _fields_ = [
+        ("field_energy_e", ctypes.c_double), ("field_energy_b", ctypes.c_double),
+        ("field_energy_b1", ctypes.c_double),
+        ("kinetic", ctypes.c_double * 2),
+        ("momentum", (ctypes.c_double * 3) * 2),
+        ("charge", ctypes.c_double), ("gauss_l2", ctypes.c_double),
+        ("gauss_max", ctypes.c_double), ("divb_l2", ctypes.c_double),
+        ("divb_max", ctypes.c_double), ("count", ctypes.c_int64 * 2),
+    ]
+
+    to_dict = PicDiag.to_dict
+
+
+FIELDS2D = FIELDS + ["phi"]
+FIELD2D_ID = {name: i for i, name in enumerate(FIELDS2D)}
+
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I64 = ctypes.c_int64
@@ -121,6 +167,44 @@ SIGNATURES = [
     ("pic_timing_enable", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_timing_get", ctypes.c_int, [_P, ctypes.c_int, _D, _I64P]),
     ("pic_timing_reset", ctypes.c_int, [_P]),
+    # include/plasmasim2d.h
+    ("pic2d_sizeof_config", ctypes.c_size_t, []),
+    ("pic2d_sizeof_diag", ctypes.c_size_t, []),
+    ("pic2d_config_preset", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Pic2dConfig)]),
+    ("pic2d_create", ctypes.c_int, [ctypes.POINTER(Pic2dConfig), ctypes.POINTER(_P)]),
+    ("pic2d_destroy", None, [_P]),
+    ("pic2d_last_error", ctypes.c_char_p, [_P]),
+    ("pic2d_get_config", ctypes.c_int, [_P, ctypes.POINTER(Pic2dConfig)]),
+    ("pic2d_stream", _P, [_P]),
+    ("pic2d_sync", ctypes.c_int, [_P]),
+    ("pic2d_upload_field", ctypes.c_int, [_P, ctypes.c_int, _D]),
+    ("pic2d_download_field", ctypes.c_int, [_P, ctypes.c_int, _D]),
+    ("pic2d_upload_particles", ctypes.c_int, [_P, ctypes.c_int, _I64] + [_D] * 5),
+    ("pic2d_download_particles", ctypes.c_int,
+     [_P, ctypes.c_int, _I64, _I64P] + [_D] * 5),
+    ("pic2d_particle_count", ctypes.c_int, [_P, ctypes.c_int, _I64P]),
+    ("pic2d_load_particles", ctypes.c_int, [_P]),
+    ("pic2d_begin", ctypes.c_int, [_P]),
+    ("pic2d_renew_grid", ctypes.c_int, [_P]),
+    ("pic2d_inflow", ctypes.c_int, [_P]),
+    ("pic2d_renew_b_half", ctypes.c_int, [_P]),
+    ("pic2d_halo", ctypes.c_int, [_P]),
+    ("pic2d_push_deposit_j", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic2d_fold_current", ctypes.c_int, [_P]),
+    ("pic2d_external_current", ctypes.c_int, [_P, _I64]),
+    ("pic2d_drift_deposit_n", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic2d_fold_density", ctypes.c_int, [_P]),
+    ("pic2d_renew_e", ctypes.c_int, [_P]),
+    ("pic2d_poisson", ctypes.c_int, [_P]),
+    ("pic2d_particle_boundary", ctypes.c_int, [_P]),
+    ("pic2d_sort", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic2d_step", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic2d_step_index", ctypes.c_int, [_P, _I64P]),
+    ("pic2d_diagnose", ctypes.c_int, [_P, ctypes.POINTER(Pic2dDiag)]),
+    ("pic2d_probe", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    ("pic2d_timing_enable", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic2d_timing_get", ctypes.c_int, [_P, ctypes.c_int, _D, _I64P]),
+    ("pic2d_timing_reset", ctypes.c_int, [_P]),
 ]
 
 
@@ -137,6 +221,10 @@ def load(path=LIB_PATH):
         raise ImportError("pic_config layout mismatch between header and binding")
     if lib.pic_sizeof_diag() != ctypes.sizeof(PicDiag):
         raise ImportError("pic_diag layout mismatch between header and binding")
+    if lib.pic2d_sizeof_config() != ctypes.sizeof(Pic2dConfig):
+        raise ImportError("pic2d_config layout mismatch between header and binding")
+    if lib.pic2d_sizeof_diag() != ctypes.sizeof(Pic2dDiag):
+        raise ImportError("pic2d_diag layout mismatch between header and binding")
     return lib
 
 

@@ -11,7 +11,8 @@ import subprocess
 
 import numpy as np
 
-from plasmasim_amd._lib import FIELD_ID, PicConfig, PicDiag, dptr
+from plasmasim_amd._lib import (FIELD_ID, FIELD2D_ID, Pic2dConfig, Pic2dDiag,
+                                PicConfig, PicDiag, dptr)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -50,6 +51,37 @@ _SIG = {
     "orc_gather": (None, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
     "orc_boris": (None, [ctypes.POINTER(PicConfig), ctypes.c_double, _D, _D]),
     "orc_sigma": (ctypes.c_double, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    # 2D3V restatement (oracle/pic2d_oracle.h)
+    "orc2d_create": (_P, [ctypes.POINTER(Pic2dConfig)]),
+    "orc2d_destroy": (None, [_P]),
+    "orc2d_upload_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "orc2d_download_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "orc2d_upload_particles": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64] + [_D] * 5),
+    "orc2d_download_particles": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64,
+                                                ctypes.POINTER(ctypes.c_int64)] + [_D] * 5),
+    "orc2d_particle_count": (ctypes.c_int64, [_P, ctypes.c_int]),
+    "orc2d_load_particles": (ctypes.c_int, [_P]),
+    "orc2d_begin": (ctypes.c_int, [_P]),
+    "orc2d_renew_grid": (ctypes.c_int, [_P]),
+    "orc2d_inflow": (ctypes.c_int, [_P]),
+    "orc2d_renew_b_half": (ctypes.c_int, [_P]),
+    "orc2d_halo": (ctypes.c_int, [_P]),
+    "orc2d_push_deposit_j": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc2d_fold_current": (ctypes.c_int, [_P]),
+    "orc2d_external_current": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "orc2d_drift_deposit_n": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc2d_fold_density": (ctypes.c_int, [_P]),
+    "orc2d_renew_e": (ctypes.c_int, [_P]),
+    "orc2d_poisson": (ctypes.c_int, [_P]),
+    "orc2d_particle_boundary": (ctypes.c_int, [_P]),
+    "orc2d_sort": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc2d_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "orc2d_diagnose": (ctypes.c_int, [_P, ctypes.POINTER(Pic2dDiag)]),
+    "orc2d_probe": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    "orc2d_gather": (None, [_P, ctypes.c_double, ctypes.c_double, _D]),
+    "orc2d_boris": (None, [ctypes.POINTER(Pic2dConfig), ctypes.c_double, _D, _D]),
+    "orc2d_dst": (None, [ctypes.c_int, _D, _D]),
+    "orc2d_sigma": (ctypes.c_double, [_P, ctypes.c_int, ctypes.c_int]),
 }
 _LIB = None
 
@@ -57,9 +89,10 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
-        src = os.path.join(ORACLE_DIR, "pic_oracle.c")
+        srcs = [os.path.join(ORACLE_DIR, f) for f in
+                ("pic_oracle.c", "pic2d_oracle.c", "pic_oracle.h", "pic2d_oracle.h")]
         if (not os.path.exists(ORACLE_SO)
-                or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src)):
+                or os.path.getmtime(ORACLE_SO) < max(os.path.getmtime(f) for f in srcs)):
             subprocess.run(["make", "-C", ORACLE_DIR], check=True,
                            capture_output=True)
         _LIB = ctypes.CDLL(ORACLE_SO)
@@ -155,6 +188,102 @@ class Oracle:
     def __getattr__(self, name):
         # routine methods: load_particles, renew_grid, ..., step(n), sort(sp)
         fn = getattr(self._L, "orc_" + name, None)
+        if fn is None:
+            raise AttributeError(name)
+
+        def call(*args):
+            self._check(fn(self._o, *args))
+        return call
+
+
+def boris2d(cfg, m, fields, p):
+    f = np.ascontiguousarray(fields, dtype=np.float64)
+    s = np.ascontiguousarray(p, dtype=np.float64).copy()
+    lib().orc2d_boris(ctypes.byref(cfg), m, dptr(f), dptr(s))
+    return s
+
+
+def dst(y):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty_like(y)
+    lib().orc2d_dst(len(y), dptr(y), dptr(out))
+    return out
+
+
+class Oracle2D:
+    """Same surface as plasmasim_amd.Simulation2D, backed by the CPU oracle."""
+
+    def __init__(self, cfg, threads=1):
+        self._L = lib()
+        self._L.orc_threads(threads)
+        self.cfg = cfg.copy()
+        self._o = self._L.orc2d_create(ctypes.byref(self.cfg))
+        if not self._o:
+            raise MemoryError("orc2d_create failed")
+        self.shape = (cfg.nx + 4, cfg.nz + 4)
+
+    def close(self):
+        if self._o:
+            self._L.orc2d_destroy(self._o)
+            self._o = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"oracle error {rc}")
+
+    def upload_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape
+        self._check(self._L.orc2d_upload_field(self._o, FIELD2D_ID[name], dptr(a)))
+
+    def download_field(self, name):
+        a = np.empty(self.shape)
+        self._check(self._L.orc2d_download_field(self._o, FIELD2D_ID[name], dptr(a)))
+        return a
+
+    def upload_particles(self, sp, parts):
+        p = np.asarray(parts, dtype=np.float64)
+        cols = [np.ascontiguousarray(p[:, j]) for j in range(5)]
+        self._check(self._L.orc2d_upload_particles(self._o, sp, len(p),
+                                                   *[dptr(c) for c in cols]))
+
+    def download_particles(self, sp):
+        n = self._L.orc2d_particle_count(self._o, sp)
+        cols = [np.empty(max(n, 1)) for _ in range(5)]
+        nn = ctypes.c_int64()
+        self._check(self._L.orc2d_download_particles(
+            self._o, sp, max(n, 1), ctypes.byref(nn), *[dptr(c) for c in cols]))
+        return np.stack([c[:nn.value] for c in cols], axis=1)
+
+    def particle_count(self, sp):
+        return self._L.orc2d_particle_count(self._o, sp)
+
+    def gather(self, x, z):
+        out = np.empty(6)
+        self._L.orc2d_gather(self._o, x, z, dptr(out))
+        return out
+
+    def sigma(self, k2, m2):
+        return self._L.orc2d_sigma(self._o, k2, m2)
+
+    def diagnose(self):
+        d = Pic2dDiag()
+        self._check(self._L.orc2d_diagnose(self._o, ctypes.byref(d)))
+        return d.to_dict()
+
+    def probe(self, k, m):
+        out = np.empty(8)
+        self._check(self._L.orc2d_probe(self._o, k, m, dptr(out)))
+        return out
+
+    def __getattr__(self, name):
+        fn = getattr(self._L, "orc2d_" + name, None)
         if fn is None:
             raise AttributeError(name)
 

@@ -1,5 +1,6 @@
 """The C-ABI library loads and exports every function include/plasmasim.h
-declares; struct layouts agree with the binding.  No GPU calls."""
+and include/plasmasim2d.h declare; struct layouts agree with the binding.
+No GPU calls."""
 import ctypes
 import os
 import re
@@ -17,7 +18,7 @@ def declared_functions():
             continue
         text = open(os.path.join(ROOT, "include", hdr)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(pic_\w+)\s*\(", text, re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(pic(?:2d)?_\w+)\s*\(", text, re.M):
             names.append(m.group(1))
     return sorted(set(names))
 
@@ -39,6 +40,21 @@ def test_struct_layout_matches():
     lib = plasmasim_amd.lib()
     assert lib.pic_sizeof_config() == ctypes.sizeof(_lib.PicConfig)
     assert lib.pic_sizeof_diag() == ctypes.sizeof(_lib.PicDiag)
+    assert lib.pic2d_sizeof_config() == ctypes.sizeof(_lib.Pic2dConfig)
+    assert lib.pic2d_sizeof_diag() == ctypes.sizeof(_lib.Pic2dDiag)
+
+
+def test_2d_presets_fill_config():
+    for n in (0, 1, 3):
+        c = plasmasim_amd.preset2d(n)
+        assert c.nx > 0 and c.ppc > 0 and c.dt < 6 / (7 * 2 ** 0.5)
+        wpe = (c.ppc * c.weight[1] * c.q ** 2 / (c.eps0 * c.mass[1])) ** 0.5
+        assert abs(wpe * c.dt - 0.1) < 1e-12
+        # line currents straddle the centre, 5 cells apart along z
+        assert c.ext_m[1] - c.ext_m[0] == 5
+        assert 2 <= c.phi_lo[1] < c.phi_hi[1] <= c.nz + 2
+    c3 = plasmasim_amd.preset2d(3)
+    assert (c3.nx, c3.nz, c3.ppc) == (2048, 2048, 100)
 
 
 def test_presets_fill_config():
