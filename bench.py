@@ -1,11 +1,15 @@
-"""Throughput of the 3D EM-PIC step on one MI355X.
+"""Throughput of the EM-PIC step on one MI355X.
 
-Workload: 128^3 cells x 64 particles/cell/species (268,435,456 particles),
-two species, dipole standoff 32 cells, inflow every 5 steps, cell sort every
-20 steps, synthetic seeded plasma.  A step is one full fdtd() step (B half
-steps, fused gather/push/deposit pass per species, current fold, E step,
-particle boundary, and the amortised inflow and sort when due), with all
-state resident in HBM.
+Default workload (3d): 128^3 cells x 64 particles/cell/species (268,435,456
+particles), two species, dipole standoff 32 cells, inflow every 5 steps,
+cell sort every 20 steps, synthetic seeded plasma.  A step is one full
+fdtd() step (B half steps, fused gather/push/deposit pass per species,
+current fold, E step, particle boundary, and the amortised inflow and sort
+when due), with all state resident in HBM.
+
+--workload 2d3v: the 2D3V variant, 2048 x 2048 cells x 100 particles/cell/
+species (838,860,800 particles), line-current dipole, DST Poisson
+correction every step, inflow every 250 steps, tile sort every 20.
 
 Prints one JSON line: value = particle-steps/s of the chosen form (FP32 by
 default), with the plain FP64 form's rate, the roofline of the dominant
@@ -22,20 +26,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# algorithmic bytes per particle-step of the fused pass (read + write state)
+# algorithmic bytes per particle-step of the fused pass (read + write state):
+# 3D 6 doubles / int32 cell + 6 floats, 2D 5 doubles / int32 cell + 5 floats
 BYTES_PER_PARTICLE = {"fp64": 96, "fp32": 56}
+BYTES_PER_PARTICLE_2D = {"fp64": 80, "fp32": 48}
 # HBM bytes per particle-step of the same pass measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE at this workload, scaled by the counts of a
 # 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md)
 TRAFFIC_PER_PARTICLE = {"fp64": 103.4, "fp32": 62.2}
 
 
-def run_form(ps, form, steps, warmup, nz=None):
-    c = ps.preset(2)
-    c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
-    if nz:
-        c.nz = nz
-    sim = ps.Simulation(c)
+def run_form(ps, form, steps, warmup, workload="3d"):
+    if workload == "2d3v":
+        c = ps.preset2d(3)
+        c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
+        sim = ps.Simulation2D(c)
+    else:
+        c = ps.preset(2)
+        c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
+        sim = ps.Simulation(c)
     sim.load_particles()
     sim.begin()
     # live per-kernel timing over every step of the run (HIP events on the
@@ -60,12 +69,15 @@ def run_form(ps, form, steps, warmup, nz=None):
     sim.close()
     per_launch_particles = particles / 2.0  # one launch per species
     push_avg_s = push_ms / push_n / 1e3
-    achieved = BYTES_PER_PARTICLE[form] * per_launch_particles / push_avg_s
+    bpp = (BYTES_PER_PARTICLE_2D if workload == "2d3v" else BYTES_PER_PARTICLE)[form]
+    achieved = bpp * per_launch_particles / push_avg_s
     return {
         "rate": rate, "ms_per_step": (t1 - t0) / steps * 1e3,
         "particles": particles, "push_avg_ms": push_avg_s * 1e3,
         "achieved": achieved, "kernel_ms": {k: v[0] / tsteps for k, v in tot.items()},
-        "traffic": TRAFFIC_PER_PARTICLE[form] * per_launch_particles,
+        "traffic": (TRAFFIC_PER_PARTICLE[form] * per_launch_particles
+                    if workload == "3d" else None),
+        "bytes_per_particle": bpp,
     }
 
 
@@ -113,12 +125,42 @@ def calibrate_copy(mib):
     torch.cuda.empty_cache()
 
 
+def cpu_baseline_2d(ps):
+    """The 2D C oracle (FP64, OpenMP) on a 2048 x 64 strip of the 2D
+    workload (1/32 of the domain, same density), 3 timed steps after 1."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle2D
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    c = ps.preset2d(3)
+    c.precision = ps.PIC_FP64
+    c.nz = 64
+    c.absorb[1] = 4
+    c.ballistic_lo[1], c.ballistic_hi[1] = 4.0, 64.0
+    c.phi_lo[1], c.phi_hi[1] = 8, 60
+    c.ext_m[0], c.ext_m[1] = 31, 36
+    o = Oracle2D(c, threads=cores)
+    o.load_particles()
+    o.begin()
+    o.step(1)
+    n = o.particle_count(0) + o.particle_count(1)
+    steps = 3
+    t0 = time.perf_counter()
+    o.step(steps)
+    t1 = time.perf_counter()
+    o.close()
+    return {"value": n * steps / (t1 - t0), "unit": "particle-steps/s",
+            "cores": cores, "kind": "port",
+            "sample": f"2D C oracle FP64, 2048x64 strip x 100 ppc x 2 species "
+                      f"({n} particles), {steps} steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--workload", choices=["3d", "2d3v"], default="3d")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -127,17 +169,26 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(one subdomain per GPU; see DESIGN.md for how it would shard)",
-              file=sys.stderr)
+              "(one subdomain per GPU)", file=sys.stderr)
         sys.exit(2)
     if a.calibrate_copy > 0:
         calibrate_copy(a.calibrate_copy)
     import plasmasim_amd as ps
-    main_r = run_form(ps, a.form, a.steps, a.warmup)
+    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload)
     other = None
     if a.form == "fp32" and not a.no_fp64:
-        other = run_form(ps, "fp64", a.steps, a.warmup)
-    cpu = None if a.no_cpu else cpu_baseline(ps)
+        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload)
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(ps) if a.workload == "3d" else cpu_baseline_2d(ps)
+    if a.workload == "2d3v":
+        workload = ("2D3V EM-PIC step, 2048^2 cells x 100 ppc x 2 species "
+                    "(838,860,800 particles), line-current dipole, dt 0.5, "
+                    "Poisson every step, sort every 20, inflow every 250")
+    else:
+        workload = ("3D EM-PIC step, 128^3 cells x 64 ppc x 2 species "
+                    "(268,435,456 particles), dipole standoff 32, dt 0.45, "
+                    "sort every 20, inflow every 5")
     out = {
         "metric": "particle-steps/s on 1 MI355X, FP32 vs FP64 form at stated field error",
         "value": main_r["rate"],
@@ -151,17 +202,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if a.form == "fp32" else "f64",
         "data": "synthetic seeded plasma (Philox4x32-10), random-free fields at start",
-        "config": {"workload": "3D EM-PIC step, 128^3 cells x 64 ppc x 2 species "
-                               "(268,435,456 particles), dipole standoff 32, dt 0.45, "
-                               "sort every 20, inflow every 5",
-                   "form": a.form},
+        "config": {"workload": workload, "form": a.form},
         "roofline": {"bound": "hbm", "achieved": main_r["achieved"] / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": main_r["achieved"] / HBM_PEAK,
                      "traffic": main_r["traffic"],
                      "kernel": "fused particle pass (gather, push, J and n deposit, boundary)",
                      "kernel_avg_ms": main_r["push_avg_ms"],
-                     "bytes_per_particle": BYTES_PER_PARTICLE[a.form]},
+                     "bytes_per_particle": main_r["bytes_per_particle"]},
         "cpu_baseline": cpu,
         "kernel_ms_per_step": main_r["kernel_ms"],
     }

@@ -766,31 +766,38 @@ int orc2d_poisson(orc2d_ctx *o) {
             (G(o, k, m + 1).ez - G(o, k, m).ez) / dx);
   /* forward DST along z per x line, then along x per z mode (fft_z,
    * fft_x); divide by the 5-point symbol; inverse along x, then z */
-  int nmax = NX > NZ ? NX : NZ;
-  double *line = (double *)malloc(sizeof(double) * (size_t)nmax);
-  double *tr = (double *)malloc(sizeof(double) * (size_t)nmax);
-  for (int k = 0; k < NX; k++) {
-    orc2d_dst(NZ, rho + (size_t)k * NZ, tr);
-    memcpy(rho + (size_t)k * NZ, tr, sizeof(double) * NZ);
-  }
-  for (int m = 0; m < NZ; m++) {
-    for (int k = 0; k < NX; k++) line[k] = rho[(size_t)k * NZ + m];
-    orc2d_dst(NX, line, tr);
+  const int nmax = NX > NZ ? NX : NZ;
+  /* the lines of each pass are independent: one per OpenMP iteration */
+#pragma omp parallel num_threads(nthreads())
+  {
+    double *line = (double *)malloc(sizeof(double) * (size_t)nmax);
+    double *tr = (double *)malloc(sizeof(double) * (size_t)nmax);
+#pragma omp for schedule(static)
     for (int k = 0; k < NX; k++) {
-      const double A = -(pow(sin(Pi * (double)m / (double)NZ / 2.), 2) +
-                         pow(sin(Pi * (double)k / (double)NX / 2.), 2)) /
-                       pow(dx / 2., 2);
-      line[k] = A == 0. ? 0. : tr[k] / A;
+      orc2d_dst(NZ, rho + (size_t)k * NZ, tr);
+      memcpy(rho + (size_t)k * NZ, tr, sizeof(double) * NZ);
     }
-    orc2d_dst(NX, line, tr);
-    for (int k = 0; k < NX; k++) rho[(size_t)k * NZ + m] = 2. * tr[k] / NX;
+#pragma omp for schedule(static)
+    for (int m = 0; m < NZ; m++) {
+      for (int k = 0; k < NX; k++) line[k] = rho[(size_t)k * NZ + m];
+      orc2d_dst(NX, line, tr);
+      for (int k = 0; k < NX; k++) {
+        const double A = -(pow(sin(Pi * (double)m / (double)NZ / 2.), 2) +
+                           pow(sin(Pi * (double)k / (double)NX / 2.), 2)) /
+                         pow(dx / 2., 2);
+        line[k] = A == 0. ? 0. : tr[k] / A;
+      }
+      orc2d_dst(NX, line, tr);
+      for (int k = 0; k < NX; k++) rho[(size_t)k * NZ + m] = 2. * tr[k] / NX;
+    }
+#pragma omp for schedule(static)
+    for (int k = 0; k < NX; k++) {
+      orc2d_dst(NZ, rho + (size_t)k * NZ, tr);
+      for (int m = 0; m < NZ; m++) rho[(size_t)k * NZ + m] = 2. * tr[m] / NZ;
+    }
+    free(line);
+    free(tr);
   }
-  for (int k = 0; k < NX; k++) {
-    orc2d_dst(NZ, rho + (size_t)k * NZ, tr);
-    for (int m = 0; m < NZ; m++) rho[(size_t)k * NZ + m] = 2. * tr[m] / NZ;
-  }
-  free(line);
-  free(tr);
   /* E += grad phi; the x neighbour of k = 0 is the periodic image
    * (:2747-2764) */
   for (int k = 0; k < NX; k++)
