@@ -23,7 +23,7 @@
  * Unpinned parts (no reference test, fixture or closed form covers them):
  * the softened dipole core (r < dipole_soft), a config-level extension.
  */
-#include "pic_oracle.h"
+#include "pic_oracle_internal.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -32,56 +32,12 @@
 #include <omp.h>
 #endif
 
-/* Grid cell record, as the reference's Grid (:115-123). */
-typedef struct {
-  double bx, by, bz;
-  double b0x, b0y, b0z;
-  double ex, ey, ez;
-  double jix0, jiy0, jiz0;
-  double jex0, jey0, jez0;
-  double ni, ne;
-  double phi;
-} Grid;
-
-/* Thread-private accumulator (reference Grid_thread, :125-129), reduced to
- * the four quantities one deposit call writes. */
-typedef struct {
-  double a[4];
-} Acc;
-
-typedef struct {
-  double x, y, z, vx, vy, vz;
-} Particle;
-
-typedef struct {
-  Particle *p;
-  int64_t n, cap;
-} PList;
-
-struct orc_ctx {
-  pic_config cfg;
-  int NX, NY, NZ;          /* interior sizes */
-  int SX, SY, SZ;          /* sizes with guards */
-  Grid *grid;
-  double *sigma;           /* 2SX x 2SY x 2SZ (reference sigma :148) */
-  PList sp[2];
-  int nthreads;
-  Acc *acc;                /* nthreads x cells */
-  int64_t step;            /* c */
-  int c_add, c_sort;       /* reference c_add, c_thrust */
-  uint32_t inflow_event;   /* Philox event id of the next inflow */
-};
-
 static int g_threads = 1;
 
 int orc_threads(int n) {
   if (n > 0) g_threads = n;
   return g_threads;
 }
-
-#define G(o, k, l, m) ((o)->grid[(((size_t)(k) * (o)->SY) + (l)) * (o)->SZ + (m)])
-#define SIG(o, k, l, m) \
-  ((o)->sigma[(((size_t)(k) * 2 * (o)->SY) + (l)) * 2 * (o)->SZ + (m)])
 
 /* ------------------------------------------------------------------ */
 /* Philox4x32-10 (Salmon et al., SC'11), the published round function. */
@@ -129,9 +85,9 @@ static inline double u53(uint32_t a, uint32_t b) {
  * units, velocity mu + sigma*N(0,1) per component (ran_multivar_gaussian
  * with a diagonal covariance, :7956-7985; Cholesky of diag(s^2) is diag(s)).
  * Normals by Box-Muller. */
-static void draw_particle(uint64_t seed, int sp, uint32_t event, uint64_t idx,
-                          double cx, double cy, double cz, double dx,
-                          double vth, const double mu[3], Particle *p) {
+void orc_draw_particle(uint64_t seed, int sp, uint32_t event, uint64_t idx,
+                       double cx, double cy, double cz, double dx,
+                       double vth, const double mu[3], Particle *p) {
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t w[4][4];
   for (int d = 0; d < 4; d++) {
@@ -163,7 +119,7 @@ static void draw_particle(uint64_t seed, int sp, uint32_t event, uint64_t idx,
 /* ------------------------------------------------------------------ */
 /* particle lists */
 
-static int plist_reserve(PList *L, int64_t cap) {
+int orc_plist_reserve(PList *L, int64_t cap) {
   if (cap <= L->cap) return 0;
   Particle *np = (Particle *)realloc(L->p, (size_t)cap * sizeof(Particle));
   if (!np) return -1;
@@ -286,6 +242,7 @@ void orc_destroy(orc_ctx *o) {
   free(o->acc);
   free(o->sp[0].p);
   free(o->sp[1].p);
+  free(o->adjG);
   free(o);
 }
 
@@ -338,7 +295,7 @@ int orc_upload_particles(orc_ctx *o, int sp, int64_t n, const double *x,
                          const double *vy, const double *vz) {
   if (sp < 0 || sp > 1 || n < 0) return PIC_E_ARG;
   PList *L = &o->sp[sp];
-  if (plist_reserve(L, n > 0 ? n : 1)) return PIC_E_NOMEM;
+  if (orc_plist_reserve(L, n > 0 ? n : 1)) return PIC_E_NOMEM;
   for (int64_t i = 0; i < n; i++) {
     Particle p = {x[i], y[i], z[i], vx[i], vy[i], vz[i]};
     L->p[i] = p;
@@ -377,7 +334,7 @@ int orc_load_particles(orc_ctx *o) {
   const int64_t n = ncell * c->ppc;
   for (int sp = 0; sp < 2; sp++) {
     PList *L = &o->sp[sp];
-    if (plist_reserve(L, n > 0 ? n : 1)) return PIC_E_NOMEM;
+    if (orc_plist_reserve(L, n > 0 ? n : 1)) return PIC_E_NOMEM;
     L->n = n;
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int64_t i = 0; i < n; i++) {
@@ -385,7 +342,7 @@ int orc_load_particles(orc_ctx *o) {
       int m = (int)(cell % o->NZ);
       int l = (int)((cell / o->NZ) % o->NY);
       int k = (int)(cell / ((int64_t)o->NZ * o->NY));
-      draw_particle(c->seed, sp, 0u, (uint64_t)i, (k + 2) * c->dx,
+      orc_draw_particle(c->seed, sp, 0u, (uint64_t)i, (k + 2) * c->dx,
                     (l + 2) * c->dx, (m + 2) * c->dx, c->dx, c->vth[sp],
                     c->vdrift, &L->p[i]);
     }
@@ -981,7 +938,7 @@ int orc_inflow(orc_ctx *o) {
     L->n = w;
     const int nface = c->refill_far ? 2 : 1;
     const int64_t per_face = (int64_t)o->NX * o->NY * 2 * c->ppc;
-    if (plist_reserve(L, L->n + per_face * nface)) return PIC_E_NOMEM;
+    if (orc_plist_reserve(L, L->n + per_face * nface)) return PIC_E_NOMEM;
     int64_t base = L->n;
     for (int face = 0; face < nface; face++) {
 #pragma omp parallel for num_threads(g_threads) schedule(static)
@@ -993,7 +950,7 @@ int orc_inflow(orc_ctx *o) {
         (void)j;
         int64_t idx = face * per_face + b;
         double zlo = face == 0 ? (m + 2) * dx : (m + o->NZ) * dx;
-        draw_particle(c->seed, sp, ev, (uint64_t)idx, (k + 2) * dx,
+        orc_draw_particle(c->seed, sp, ev, (uint64_t)idx, (k + 2) * dx,
                       (l + 2) * dx, zlo, dx, c->vth[sp], c->vdrift,
                       &L->p[base + idx]);
       }

@@ -62,6 +62,20 @@ void orc_boris(const pic_config *cfg, double m, const double f[6], double p[6]);
 /* sigma on the 2x lattice */
 double orc_sigma(orc_ctx *o, int k2, int l2, int m2);
 
+/* FP64 discrete adjoint (pic_oracle_adj.c).  p = (Mx, My, Mz, V).  The
+ * forward is load, inflow and nsteps steps without the sort; the loss is
+ * the step average of the Maxwell-stress z flux through a sphere of R
+ * cells (nq quadrature points) around the dipole. */
+double orc_adj_loss(orc_ctx *o, const double p[4], int nsteps, double R, int nq);
+int orc_adj_gradient(orc_ctx *o, const double p[4], int nsteps, double R,
+                     int nq, double *loss, double grad[4]);
+/* stress flux through the sphere for the current fields */
+double orc_adj_force(orc_ctx *o, double R, int nq);
+/* dot-product checks of the transposed field operators */
+int orc_adj_selftest(orc_ctx *o, uint64_t seed, double out[4]);
+int orc_adj_boris_check(const pic_config *cfg, double m, const double f[6],
+                        const double v[3], const double lvp[3], double out[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,14 @@ _SIG = {
     "orc_gather": (None, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
     "orc_boris": (None, [ctypes.POINTER(PicConfig), ctypes.c_double, _D, _D]),
     "orc_sigma": (ctypes.c_double, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    # FP64 adjoint (oracle/pic_oracle_adj.c)
+    "orc_adj_loss": (ctypes.c_double, [_P, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "orc_adj_gradient": (ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        _D, _D]),
+    "orc_adj_force": (ctypes.c_double, [_P, ctypes.c_double, ctypes.c_int]),
+    "orc_adj_selftest": (ctypes.c_int, [_P, ctypes.c_uint64, _D]),
+    "orc_adj_boris_check": (ctypes.c_int, [ctypes.POINTER(PicConfig), ctypes.c_double,
+                                           _D, _D, _D, _D]),
     # 2D3V restatement (oracle/pic2d_oracle.h)
     "orc2d_create": (_P, [ctypes.POINTER(Pic2dConfig)]),
     "orc2d_destroy": (None, [_P]),
@@ -90,7 +98,8 @@ def lib():
     global _LIB
     if _LIB is None:
         srcs = [os.path.join(ORACLE_DIR, f) for f in
-                ("pic_oracle.c", "pic2d_oracle.c", "pic_oracle.h", "pic2d_oracle.h")]
+                ("pic_oracle.c", "pic_oracle_adj.c", "pic2d_oracle.c", "pic_oracle.h",
+                 "pic_oracle_internal.h", "pic2d_oracle.h")]
         if (not os.path.exists(ORACLE_SO)
                 or os.path.getmtime(ORACLE_SO) < max(os.path.getmtime(f) for f in srcs)):
             subprocess.run(["make", "-C", ORACLE_DIR], check=True,
@@ -184,6 +193,27 @@ class Oracle:
         d = PicDiag()
         self._check(self._L.orc_diagnose(self._o, ctypes.byref(d)))
         return d.to_dict()
+
+    # FP64 adjoint: p = (Mx, My, Mz, V)
+    def adj_loss(self, p, nsteps, R, nq):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        return self._L.orc_adj_loss(self._o, dptr(p), nsteps, R, nq)
+
+    def adj_gradient(self, p, nsteps, R, nq):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        g = np.zeros(4)
+        loss = ctypes.c_double()
+        self._check(self._L.orc_adj_gradient(self._o, dptr(p), nsteps, R, nq,
+                                             ctypes.byref(loss), dptr(g)))
+        return loss.value, g
+
+    def adj_force(self, R, nq):
+        return self._L.orc_adj_force(self._o, R, nq)
+
+    def adj_selftest(self, seed=1):
+        out = np.zeros(4)
+        self._check(self._L.orc_adj_selftest(self._o, seed, dptr(out)))
+        return out
 
     def __getattr__(self, name):
         # routine methods: load_particles, renew_grid, ..., step(n), sort(sp)
