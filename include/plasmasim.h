@@ -229,6 +229,28 @@ int pic_timing_enable(pic_ctx *ctx, int on);
 int pic_timing_get(pic_ctx *ctx, int which, double *ms, int64_t *launches);
 int pic_timing_reset(pic_ctx *ctx);
 
+/*
+ * Adjoint of the step (no counterpart in the reference).  Gradient of the
+ * loss L = (1/N) sum over the N steps of the z (flow-direction) flux of the
+ * Maxwell stress tensor, eps0 (E_z E.n - n_z |E|^2/2) + (B_z B.n - n_z
+ * |B|^2/2)/mu0 with the total B, through a sphere of `radius` cells around
+ * the dipole (nquad Fibonacci points, equal weights), with respect to
+ * p = (Mx, My, Mz, V): the dipole moment vector (B0 = M-field + IMF) and
+ * the inflow drift speed along z.
+ *
+ * pic_adj_forward sets B0 from M and V, loads the particles and runs nsteps
+ * steps of the context's form with the sort off (particles keep their
+ * slots; the capacity must hold every injected particle), checkpointing
+ * the state every ckpt_every steps; it returns L.  pic_adj_backward
+ * recomputes each segment from its checkpoint and reverses it in FP64
+ * (transposed FDTD, guard copies, fold, gather, Boris step and deposit),
+ * returning dloss * dL/dp.  The context is left at the end of the last
+ * recomputed segment.
+ */
+int pic_adj_forward(pic_ctx *ctx, const double params[4], int nsteps,
+                    int ckpt_every, double radius, int nquad, double *loss);
+int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -177,6 +177,21 @@ class Simulation:
     def stream(self):
         return self._L.pic_stream(self._ctx)
 
+    # -- adjoint ------------------------------------------------------------
+    def adj_forward(self, params, nsteps, ckpt_every, radius, nquad):
+        """Loss for p = (Mx, My, Mz, V); keeps checkpoints for adj_backward."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        loss = ctypes.c_double()
+        self._check(self._L.pic_adj_forward(self._ctx, _lib.dptr(p), int(nsteps),
+                                            int(ckpt_every), float(radius), int(nquad),
+                                            ctypes.byref(loss)))
+        return loss.value
+
+    def adj_backward(self, dloss=1.0):
+        g = np.zeros(4)
+        self._check(self._L.pic_adj_backward(self._ctx, float(dloss), _lib.dptr(g)))
+        return g
+
     # -- timing -------------------------------------------------------------
     def timing(self, on=True):
         self._check(self._L.pic_timing_enable(self._ctx, 1 if on else 0))

@@ -167,6 +167,9 @@ SIGNATURES = [
     ("pic_timing_enable", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_timing_get", ctypes.c_int, [_P, ctypes.c_int, _D, _I64P]),
     ("pic_timing_reset", ctypes.c_int, [_P]),
+    ("pic_adj_forward", ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, _D]),
+    ("pic_adj_backward", ctypes.c_int, [_P, ctypes.c_double, _D]),
     # include/plasmasim2d.h
     ("pic2d_sizeof_config", ctypes.c_size_t, []),
     ("pic2d_sizeof_diag", ctypes.c_size_t, []),

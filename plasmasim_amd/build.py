@@ -29,6 +29,7 @@ UNITS = {
     "pic_fp64.hip": ["-ffp-contract=off"],
     "pic_fp32.hip": ["-ffp-contract=fast"],
     "pic_prim.hip": ["-ffp-contract=off"],
+    "pic_adj.hip": ["-ffp-contract=off"],
     "pic2d_api.hip": ["-ffp-contract=off"],
     "pic2d_fields.hip": ["-ffp-contract=off"],
     "pic2d_particles.hip": ["-ffp-contract=off"],
@@ -46,7 +47,7 @@ def _compile(src, flags):
 
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=7) as ex:
+    with ThreadPoolExecutor(max_workers=8) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs

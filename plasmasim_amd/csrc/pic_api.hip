@@ -305,6 +305,7 @@ static void free_all(pic_ctx *ctx) {
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   free(ctx->b0_host);
   if (ctx->cfg.precision == PIC_FP32) f32::free_state(ctx);
+  adj_free(ctx);
 }
 
 // ---------------------------------------------------------------------------
@@ -751,6 +752,15 @@ extern "C" int pic_get_counters(const pic_ctx *ctx, int64_t *step,
   return 0;
 }
 
+namespace pic {
+int push_fused(pic_ctx *ctx, int sp) {
+  timing_begin(ctx);
+  int rc = DISPATCH(push, ctx, sp, 1, 1, 1);
+  timing_end(ctx, PIC_T_PUSH);
+  return rc;
+}
+}  // namespace pic
+
 // fdtd() step loop (:687-949) with loops A and B and the particle boundary
 // fused into one pass per species.
 extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
@@ -763,7 +773,7 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
   } while (0)
   for (int s = 0; s < nsteps; ++s) {
     TRY(pic_renew_grid(ctx));
-    if (ctx->c_sort > c.sort_every - 1) {
+    if (!ctx->adj_nosort && ctx->c_sort > c.sort_every - 1) {
       ctx->c_sort = 0;
       TRY(pic_sort(ctx, PIC_ION));
       TRY(pic_sort(ctx, PIC_ELECTRON));

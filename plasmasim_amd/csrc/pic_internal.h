@@ -114,6 +114,10 @@ struct pic_ctx {
   // deterministic accumulators)
   void *form_state = nullptr;
 
+  // adjoint: checkpoints, tape and adjoint planes (pic_adj.hip)
+  void *adj = nullptr;
+  int adj_nosort = 0;  // the adjoint's forward runs without the sort
+
   // step counters (reference fdtd(): c, c_add, c_thrust)
   int64_t step = 0;
   int32_t c_add = 0, c_sort = 0;
@@ -229,6 +233,12 @@ int refresh_total_b(pic_ctx *);
 int download_raw32(pic_ctx *, int sp, int64_t cap, int64_t *n, int32_t *cell,
                    float *const *f);
 }  // namespace f32
+
+// the fused particle pass of one species (gather, push, J and n deposit,
+// particle boundary), as pic_step runs it (pic_api.hip)
+int push_fused(pic_ctx *ctx, int sp);
+// adjoint state (pic_adj.hip)
+void adj_free(pic_ctx *ctx);
 
 // device-wide building blocks on rocPRIM (pic_prim.hip)
 // stable selection: idx_out[0..*count) = indices i < n with flags[i] != 0

@@ -128,15 +128,22 @@ def test_inflow_and_sort_bit_exact():
         s.upload_particles(0, p)
         s.upload_particles(1, p[::-1].copy())
         s.inflow()
+    # positions are integer-exact arithmetic on the same draws; velocities
+    # pass through the device's log and cos, which may differ from the
+    # host's in the last bit
     for sp in (0, 1):
-        assert np.array_equal(g.download_particles(sp), o.download_particles(sp))
+        a, b = g.download_particles(sp), o.download_particles(sp)
+        assert a.shape == b.shape
+        assert np.array_equal(a[:, :2], b[:, :2])
+        assert rel_l2(a[:, 2:], b[:, 2:]) < 1e-14
     for s in (g, o):
         s.sort(0)
         s.sort(1)
     for sp in (0, 1):
         a, b = g.download_particles(sp), o.download_particles(sp)
         assert g.particle_count(sp) == o.particle_count(sp)
-        assert np.array_equal(a, b)
+        assert np.array_equal(a[:, :2], b[:, :2])
+        assert rel_l2(a[:, 2:], b[:, 2:]) < 1e-14
 
 
 @pytest.mark.parametrize("sort_every", [0, 3])
@@ -161,8 +168,14 @@ def test_whole_steps_match_oracle(sort_every):
         assert abs(dg[k] - do[k]) <= 1e-10 * max(abs(do[k]), 1e-300), k
     assert np.allclose(dg["kinetic"], do["kinetic"], rtol=1e-12, atol=0)
     assert np.array_equal(dg["count"], do["count"])
+    # probe columns: the six fields to the field tolerance; |div B|/|B| is
+    # rounding noise (the (2,4) update keeps div B at zero), so both sides
+    # must only be at rounding level; the Gauss ratio to 1e-8
     k, m = c.ext_k, c.ext_m[1]
-    assert np.allclose(g.probe(k, m), o.probe(k, m), rtol=1e-10, atol=0)
+    pg, po = g.probe(k, m), o.probe(k, m)
+    assert np.allclose(pg[:6], po[:6], rtol=1e-10, atol=1e-12 * np.abs(po[:6]).max())
+    assert pg[6] < 1e-10 and po[6] < 1e-10
+    assert abs(pg[7] - po[7]) <= 1e-8 * abs(po[7])
 
 
 def test_fused_step_matches_routine_sequence():
