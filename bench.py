@@ -7,6 +7,12 @@ fdtd() step (B half steps, fused gather/push/deposit pass per species,
 current fold, E step, particle boundary, and the amortised inflow and sort
 when due), with all state resident in HBM.
 
+--workload adjoint: gradient of the step-averaged Maxwell-stress flux
+through a sphere of 10 cells around the dipole with respect to the dipole
+moment vector and the inflow speed, 64^3 cells x 32 ppc/species, 200 steps,
+checkpoint every 20, FP32 forward with FP64 adjoint accumulation; checked
+against the FP64-forward adjoint and central differences.
+
 --workload 2d3v: the 2D3V variant, 2048 x 2048 cells x 100 particles/cell/
 species (838,860,800 particles), line-current dipole, DST Poisson
 correction every step, inflow every 250 steps, tile sort every 20.
@@ -154,13 +160,76 @@ def cpu_baseline_2d(ps):
                       f"({n} particles), {steps} steps"}
 
 
+def run_adjoint(ps, steps=200, ckpt=20, radius=10.0, nquad=1024,
+                horizons=(20, 40, 80, 200)):
+    """Forward + backward at the adjoint workload in both forms, and the
+    FP64 gradient against central differences of the FP64 forward (h = 1e-4
+    and 1e-7 of each parameter) over growing horizons."""
+    import numpy as np
+
+    def make(form):
+        c = ps.preset(4)
+        c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
+        n0 = c.nx * c.ny * c.nz * c.ppc
+        add = c.nx * c.ny * 2 * c.ppc * (2 if c.refill_far else 1)
+        c.capacity = n0 + (steps // c.inflow_every + 2) * add
+        return c, n0
+
+    c, n0 = make("fp64")
+    M = c.dipole_m
+    # the workload's moment M (0, sin a, cos a), as a vector
+    p = np.array([0.0, M * np.sin(c.dipole_alpha), M * np.cos(c.dipole_alpha), c.vdrift[2]])
+    scale = np.array([M, M, M, abs(c.vdrift[2])])
+    out = {"params": p.tolist()}
+    for form in ("fp32", "fp64"):
+        cf, _ = make(form)
+        sim = ps.Simulation(cf)
+        sim.sync()
+        t0 = time.perf_counter()
+        loss = sim.adj_forward(p, steps, ckpt, radius, nquad)
+        sim.sync()
+        t1 = time.perf_counter()
+        g = sim.adj_backward(1.0)
+        sim.sync()
+        t2 = time.perf_counter()
+        out[form] = {"loss": loss, "grad": g.tolist(), "forward_s": t1 - t0,
+                     "backward_s": t2 - t1, "particles": 2 * n0}
+        sim.close()
+    g32, g64 = np.array(out["fp32"]["grad"]), np.array(out["fp64"]["grad"])
+    out["fp32_vs_fp64_grad_rel"] = (np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)).tolist()
+    sim = ps.Simulation(c)
+    table = []
+    for N in horizons:
+        loss = sim.adj_forward(p, N, ckpt, radius, nquad)
+        g = sim.adj_backward(1.0)
+        row = {"steps": N, "loss": loss, "adjoint": g.tolist()}
+        for h_rel in (1e-4, 1e-7):
+            fd = []
+            for i in range(4):
+                h = h_rel * scale[i]
+                pp, pm = p.copy(), p.copy()
+                pp[i] += h
+                pm[i] -= h
+                fd.append((sim.adj_forward(pp, N, N, radius, nquad) -
+                           sim.adj_forward(pm, N, N, radius, nquad)) / (2 * h))
+            row[f"fd_h{h_rel:g}"] = fd
+            row[f"rel_err_h{h_rel:g}"] = [abs(a - b) / max(abs(a), 1e-300)
+                                          for a, b in zip(fd, g.tolist())]
+        table.append(row)
+        print(f"adjoint horizon {N}: rel err (h=1e-7) {row['rel_err_h1e-07']}",
+              file=sys.stderr, flush=True)
+    sim.close()
+    out["horizons"] = table
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--workload", choices=["3d", "2d3v"], default="3d")
+    ap.add_argument("--workload", choices=["3d", "2d3v", "adjoint"], default="3d")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -174,6 +243,20 @@ def main():
     if a.calibrate_copy > 0:
         calibrate_copy(a.calibrate_copy)
     import plasmasim_amd as ps
+    if a.workload == "adjoint":
+        r = run_adjoint(ps)
+        f32 = r["fp32"]
+        print(json.dumps({
+            "metric": "adjoint particle-steps/s (forward + backward) on 1 MI355X",
+            "value": f32["particles"] * 200 / (f32["forward_s"] + f32["backward_s"]),
+            "unit": "particle-steps/s", "n_gpus": 1, "steps": 200, "warmup": 0,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 forward, f64 adjoint", "data": "synthetic seeded plasma",
+            "config": {"workload": "adjoint: 64^3 x 32 ppc x 2 species, 200 steps, "
+                                   "checkpoint every 20, stress flux through r = 10 cells, "
+                                   "1024 quadrature points"},
+            "adjoint": r}))
+        return
     main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload)
     other = None
     if a.form == "fp32" and not a.no_fp64:
