@@ -31,7 +31,21 @@
 namespace pic {
 namespace f32 {
 
-constexpr int kWin = 1536;      // LDS window capacity in nodes (4 x int64 each)
+#ifndef PIC_KWIN32
+#define PIC_KWIN32 1536
+#endif
+#ifndef PIC_MINWAVES32
+#define PIC_MINWAVES32 1
+#endif
+#ifndef PIC_PPT32
+#define PIC_PPT32 4  // particles per thread and vector width of the pass
+#endif
+#if PIC_MINWAVES32 > 1
+#define PIC_WAVES32 __attribute__((amdgpu_waves_per_eu(PIC_MINWAVES32, 8)))
+#else
+#define PIC_WAVES32
+#endif
+constexpr int kWin = PIC_KWIN32;  // LDS window capacity in nodes (4 x int64 each)
 constexpr int kPackShift = 10;
 constexpr int kPackMask = 1023;
 
@@ -336,7 +350,7 @@ __device__ __forceinline__ void particle_step(
 // the fused particle pass
 
 template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET>
-__global__ __launch_bounds__(256) void k_particles32(
+__global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
     Grid32 G, TileGeom tg, Particles32 P, TileRun tr, Push32 kp,
     typename GlobT<DET>::T *jx, typename GlobT<DET>::T *jy,
     typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd) {
@@ -373,42 +387,51 @@ __global__ __launch_bounds__(256) void k_particles32(
   for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin + i % WN] = 0;
   __syncthreads();
 
-  // each thread takes 4 consecutive particles with 16-byte loads; slots of
-  // a boundary quad that belong to a neighbouring range are left untouched
-  const int64_t q0 = s >> 2, q1 = (e + 3) >> 2;
+  // each thread takes PIC_PPT32 consecutive particles with vector loads;
+  // slots of a boundary group that belong to a neighbouring range are left
+  // untouched
+  typedef int vi __attribute__((ext_vector_type(PIC_PPT32)));
+  typedef float vf __attribute__((ext_vector_type(PIC_PPT32)));
+  constexpr int N = PIC_PPT32;
+  constexpr int SH = N == 4 ? 2 : N == 2 ? 1 : 0;
+  const int64_t q0 = s >> SH, q1 = (e + N - 1) >> SH;
   for (int64_t q = q0 + tid; q < q1; q += blockDim.x) {
-    const int64_t i0 = q << 2;
-    const int4 c4 = *(const int4 *)(P.cell + i0);
-    const float4 x4 = *(const float4 *)(P.ox + i0);
-    const float4 y4 = *(const float4 *)(P.oy + i0);
-    const float4 z4 = *(const float4 *)(P.oz + i0);
-    const float4 u4 = *(const float4 *)(P.vx + i0);
-    const float4 v4 = *(const float4 *)(P.vy + i0);
-    const float4 w4 = *(const float4 *)(P.vz + i0);
-    int cs[4] = {c4.x, c4.y, c4.z, c4.w};
-    float xs[4] = {x4.x, x4.y, x4.z, x4.w}, ys[4] = {y4.x, y4.y, y4.z, y4.w},
-          zs[4] = {z4.x, z4.y, z4.z, z4.w}, us[4] = {u4.x, u4.y, u4.z, u4.w},
-          vs[4] = {v4.x, v4.y, v4.z, v4.w}, ws[4] = {w4.x, w4.y, w4.z, w4.w};
-    const bool full = i0 >= s && i0 + 3 < e;
+    const int64_t i0 = q << SH;
+    vi cs = *(const vi *)(P.cell + i0);
+    vf xs = *(const vf *)(P.ox + i0);
+    vf ys = *(const vf *)(P.oy + i0);
+    vf zs = *(const vf *)(P.oz + i0);
+    vf us = *(const vf *)(P.vx + i0);
+    vf vs = *(const vf *)(P.vy + i0);
+    vf ws = *(const vf *)(P.vz + i0);
+    const bool full = i0 >= s && i0 + N - 1 < e;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < N; ++j) {
       const int64_t i = i0 + j;
       if (i < s || i >= e) continue;
-      particle_step<PUSH_J, DRIFT_N, BOUNDARY>(G, kp, w, acc, jx, jy, jz, nd,
-                                               cs[j], xs[j], ys[j], zs[j],
-                                               us[j], vs[j], ws[j]);
+      int c = cs[j];
+      float x = xs[j], y = ys[j], z = zs[j], u = us[j], v = vs[j], ww = ws[j];
+      particle_step<PUSH_J, DRIFT_N, BOUNDARY>(G, kp, w, acc, jx, jy, jz, nd, c, x,
+                                               y, z, u, v, ww);
+      cs[j] = c;
+      xs[j] = x;
+      ys[j] = y;
+      zs[j] = z;
+      us[j] = u;
+      vs[j] = v;
+      ws[j] = ww;
     }
     if (full) {
-      *(int4 *)(P.cell + i0) = make_int4(cs[0], cs[1], cs[2], cs[3]);
-      *(float4 *)(P.ox + i0) = make_float4(xs[0], xs[1], xs[2], xs[3]);
-      *(float4 *)(P.oy + i0) = make_float4(ys[0], ys[1], ys[2], ys[3]);
-      *(float4 *)(P.oz + i0) = make_float4(zs[0], zs[1], zs[2], zs[3]);
-      *(float4 *)(P.vx + i0) = make_float4(us[0], us[1], us[2], us[3]);
-      *(float4 *)(P.vy + i0) = make_float4(vs[0], vs[1], vs[2], vs[3]);
-      *(float4 *)(P.vz + i0) = make_float4(ws[0], ws[1], ws[2], ws[3]);
+      *(vi *)(P.cell + i0) = cs;
+      *(vf *)(P.ox + i0) = xs;
+      *(vf *)(P.oy + i0) = ys;
+      *(vf *)(P.oz + i0) = zs;
+      *(vf *)(P.vx + i0) = us;
+      *(vf *)(P.vy + i0) = vs;
+      *(vf *)(P.vz + i0) = ws;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < N; ++j) {
         const int64_t i = i0 + j;
         if (i < s || i >= e) continue;
         P.cell[i] = cs[j];
