@@ -39,7 +39,7 @@ BYTES_PER_PARTICLE_2D = {"fp64": 80, "fp32": 48}
 # HBM bytes per particle-step of the same pass measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE at this workload, scaled by the counts of a
 # 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md)
-TRAFFIC_PER_PARTICLE = {"fp64": 103.4, "fp32": 62.2}
+TRAFFIC_PER_PARTICLE = {"fp64": 101.3, "fp32": 60.3}
 
 
 def run_form(ps, form, steps, warmup, workload="3d"):

@@ -992,19 +992,6 @@ __global__ void k_keys2(Geom2 g, Tile2 tg, int f64, P64 P, P32 Q, int64_t n,
   keys[i] = key;
 }
 
-// tile_start[t] = first sorted slot of tile t; keys of dropped slots map to
-// tile nt, so tile_start[nt] is the number of kept particles
-__global__ void k_tile_bounds(const uint32_t *__restrict__ skeys, int64_t n,
-                              uint32_t cpt, int nt, uint32_t *tile_start) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int tc = (int)min((uint32_t)nt, skeys[i] / cpt);
-  const int tp = i == 0 ? -1 : (int)min((uint32_t)nt, skeys[i - 1] / cpt);
-  for (int t = tp + 1; t <= tc; ++t) tile_start[t] = (uint32_t)i;
-  if (i == n - 1)
-    for (int t = tc + 1; t <= nt; ++t) tile_start[t] = (uint32_t)n;
-}
-
 int sort(pic2d_ctx *ctx, int sp) {
   const int64_t n = ctx->n[sp];
   const int nt = ctx->tg.ntiles();
@@ -1023,9 +1010,8 @@ int sort(pic2d_ctx *ctx, int sp) {
                           n, bits);
   if (rc) return rc;
   const uint32_t cpt = (uint32_t)(ctx->tg.TX * ctx->tg.TZ);
-  k_tile_bounds<<<(unsigned)div_up(n, kBlock), kBlock, 0, st>>>(
-      ctx->rt.keys_alt, n, cpt, nt, ctx->tile_start[sp]);
-  P2_LAUNCH_CHECK(ctx);
+  rc = pic::tile_bounds(&ctx->rt, ctx->rt.keys_alt, n, cpt, nt, ctx->tile_start[sp]);
+  if (rc) return rc;
   P2_HIP(ctx, hipMemcpyAsync(ctx->rt.host_counter, ctx->tile_start[sp] + nt,
                              sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   P2_HIP(ctx, hipStreamSynchronize(st));

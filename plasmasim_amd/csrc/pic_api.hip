@@ -294,7 +294,6 @@ static void free_all(pic_ctx *ctx) {
   if (ctx->flags) (void)hipFree(ctx->flags);
   if (ctx->prim_tmp) (void)hipFree(ctx->prim_tmp);
   if (ctx->dev_counter) (void)hipFree(ctx->dev_counter);
-  if (ctx->tile_count) (void)hipFree(ctx->tile_count);
   for (int sp = 0; sp < 2; ++sp)
     if (ctx->tile_start[sp]) (void)hipFree(ctx->tile_start[sp]);
   if (ctx->host_counter) (void)hipHostFree(ctx->host_counter);
@@ -435,7 +434,6 @@ extern "C" int pic_create(const pic_config *cfg, pic_ctx **out) {
   CREATE_HIP(hipMalloc(&ctx->flags, cap));
   {
     const int nt = tile_geom(ctx).ntiles();
-    CREATE_HIP(hipMalloc(&ctx->tile_count, (size_t)nt * sizeof(uint32_t)));
     for (int sp = 0; sp < 2; ++sp) {
       CREATE_HIP(hipMalloc(&ctx->tile_start[sp], (size_t)(nt + 1) * sizeof(uint32_t)));
       CREATE_HIP(hipMemset(ctx->tile_start[sp], 0, (size_t)(nt + 1) * sizeof(uint32_t)));

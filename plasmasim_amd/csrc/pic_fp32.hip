@@ -178,17 +178,12 @@ __device__ inline void drift_axis(int &k, float &o, float v, float hdt) {
 
 typedef unsigned long long u64;
 
-// Fixed-point value of one contribution.  The scale is a power of two
-// chosen so that |v * s| < 2^31 for every contribution, hence v * s is exact
-// in fp32 and rounds to an int32 that sign-extends to the 64-bit integer.
-__device__ inline u64 to_fx(float v, float s) {
-  return (u64)(long long)(int)rintf(v * s);
-}
-
-__device__ inline void lds_add(u64 *p, float v, float s) {
-  __hip_atomic_fetch_add((PIC_LDS u64 *)p, to_fx(v, s), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+// Fixed-point value of one contribution already multiplied by the scale.
+// The scale is a power of two chosen so that every scaled contribution is
+// below 2^30 in magnitude (multiplying by it is exact); the conversion
+// truncates to an int32 (an error below one unit, 2^-30 of the largest
+// contribution) that sign-extends to the 64-bit integer.
+__device__ inline u64 to_fx(float v) { return (u64)(long long)(int)v; }
 
 // global target: fp32 planes (atomic mode) or fixed-point planes
 template <bool DET>
@@ -202,12 +197,12 @@ struct GlobT<true> {
   using T = u64;
 };
 
-__device__ inline void glob_add(float *p, float v, float) {
+__device__ inline void glob_add(float *p, float v) {
   __hip_atomic_fetch_add((PIC_GLB float *)p, v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void glob_add(u64 *p, float v, float s) {
-  __hip_atomic_fetch_add((PIC_GLB u64 *)p, to_fx(v, s), __ATOMIC_RELAXED,
+__device__ inline void glob_add(u64 *p, float v) {
+  __hip_atomic_fetch_add((PIC_GLB u64 *)p, to_fx(v), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 // flush of one LDS node
@@ -220,54 +215,177 @@ __device__ inline void glob_flush(u64 *p, u64 v, double) {
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Deposit of the 8 nodes of one trilinear stencil.  Lanes of a wave mostly
-// hold particles of the same cell (the particles are cell-sorted), so they
-// hit the same LDS nodes; lane j visits the nodes in the order n ^ (j & 7),
-// which spreads one instruction's lanes over different nodes and banks.
-#ifndef PIC_SWIZZLE
-#define PIC_SWIZZLE 1
-#endif
+// ---------------------------------------------------------------------------
+// the fused particle pass
+//
+// Per-particle work is written for a low VALU count, the limit of this pass:
+// 32-bit byte offsets from uniform plane bases for the gather (SGPR-base
+// loads), one window index per particle for the LDS deposit, and a lane-
+// constant walk of each 8-node stencil.  Lanes of a wave mostly hold
+// particles of the same cell (the particles are cell-sorted) and would hit
+// the same LDS node in the same instruction; lane j walks its stencil
+// mirrored along the axes set in j & 7, so one instruction's lanes spread
+// over the eight nodes.  The mirror costs two selects per axis and quantity;
+// the eight node offsets of the walk are per-lane constants.
+
+// the lane's walk over the 8 nodes of a stencil in the LDS window
+struct Walk {
+  bool mx, my, mz;  // axes walked downwards
+  int first;        // byte offset of the first visited node from the stencil base
+  int off[8];       // byte offsets of the visits from the first node
+};
+
+__device__ inline Walk make_walk(int WY, int WZ) {
+  Walk k;
+  const int sw = threadIdx.x & 7;
+  k.mx = sw & 4;
+  k.my = sw & 2;
+  k.mz = sw & 1;
+  const int sz = 8, sy = 8 * WZ, sx = 8 * WY * WZ;
+  const int dx = k.mx ? -sx : sx, dy = k.my ? -sy : sy, dz = k.mz ? -sz : sz;
+  k.first = (k.mx ? sx : 0) + (k.my ? sy : 0) + (k.mz ? sz : 0);
+  k.off[0] = 0;
+  k.off[1] = dz;
+  k.off[2] = dy;
+  k.off[3] = dy + dz;
+  k.off[4] = dx;
+  k.off[5] = dx + dz;
+  k.off[6] = dx + dy;
+  k.off[7] = dx + dy + dz;
+  return k;
+}
+
+__device__ __forceinline__ void lds_fx_add(char *acc, int byte, float v) {
+  __hip_atomic_fetch_add((PIC_LDS u64 *)(acc + byte), to_fx(v), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Deposit q * a_i b_j c_k at the 8 window nodes of a stencil whose lowest
+// node is at byte offset base; q carries the fixed-point scale.
+__device__ __forceinline__ void lds_dep8(char *acc, int base, const Walk &k,
+                                         float q, float a0, float a1, float b0,
+                                         float b1, float c0, float c1) {
+  const float A0 = k.mx ? a1 : a0, A1 = k.mx ? a0 : a1;
+  const float B0 = k.my ? b1 : b0, B1 = k.my ? b0 : b1;
+  const float C0 = k.mz ? c1 : c0, C1 = k.mz ? c0 : c1;
+  const float qa0 = q * A0, qa1 = q * A1;
+  const float bc00 = B0 * C0, bc01 = B0 * C1, bc10 = B1 * C0, bc11 = B1 * C1;
+  const int p = base + k.first;
+  lds_fx_add(acc, p + k.off[0], qa0 * bc00);
+  lds_fx_add(acc, p + k.off[1], qa0 * bc01);
+  lds_fx_add(acc, p + k.off[2], qa0 * bc10);
+  lds_fx_add(acc, p + k.off[3], qa0 * bc11);
+  lds_fx_add(acc, p + k.off[4], qa1 * bc00);
+  lds_fx_add(acc, p + k.off[5], qa1 * bc01);
+  lds_fx_add(acc, p + k.off[6], qa1 * bc10);
+  lds_fx_add(acc, p + k.off[7], qa1 * bc11);
+}
+
+// the same stencil straight into the global planes (particles outside the
+// window); q carries the fixed-point scale in deterministic mode only
 template <typename G>
-__device__ inline void dep8(u64 *lds, G *glob, bool in_lds, const Win &w,
-                            const Geom &g, int a, int b, int c, float a0,
-                            float a1, float b0, float b1, float c0, float c1,
-                            float q, float s) {
+__device__ __noinline__ void glob_dep8(G *glob, const Geom &g, int a, int b, int c,
+                                       float q, float a0, float a1, float b0,
+                                       float b1, float c0, float c1) {
+  const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+  const int64_t i = g.idx(a, b, c);
   const float qa0 = q * a0, qa1 = q * a1;
-  if (in_lds) {
-    const int WZ = w.WZ, WYZ = w.WY * w.WZ;
-    const int i = ((a - w.k0) * w.WY + (b - w.l0)) * WZ + (c - w.m0);
-#if PIC_SWIZZLE
-    const int sw = threadIdx.x & 7;
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-      const int id = n ^ sw;
-      const float wa = (id & 4) ? qa1 : qa0;
-      const float wb = (id & 2) ? b1 : b0;
-      const float wc = (id & 1) ? c1 : c0;
-      const int off = ((id & 4) ? WYZ : 0) + ((id & 2) ? WZ : 0) + (id & 1);
-      lds_add(lds + i + off, wa * (wb * wc), s);
+  glob_add(glob + i, qa0 * (b0 * c0));
+  glob_add(glob + i + 1, qa0 * (b0 * c1));
+  glob_add(glob + i + dl, qa0 * (b1 * c0));
+  glob_add(glob + i + dl + 1, qa0 * (b1 * c1));
+  glob_add(glob + i + dk, qa1 * (b0 * c0));
+  glob_add(glob + i + dk + 1, qa1 * (b0 * c1));
+  glob_add(glob + i + dk + dl, qa1 * (b1 * c0));
+  glob_add(glob + i + dk + dl + 1, qa1 * (b1 * c1));
+}
+
+__device__ __forceinline__ const float *at(const float *base, uint32_t byte) {
+  return (const float *)((const char *)base + byte);
+}
+
+// trilinear interpolation of one plane around the byte offset o of its
+// stencil base; sl, sk: byte strides of l and k
+__device__ __forceinline__ float lerp8o(const float *F, uint32_t o, uint32_t sl,
+                                        uint32_t sk, float a0, float a1, float b0,
+                                        float b1, float c0, float c1) {
+  const float *p00 = at(F, o), *p01 = at(F, o + sl), *p10 = at(F, o + sk),
+              *p11 = at(F, o + sk + sl);
+  const float v00 = c0 * p00[0] + c1 * p00[1];
+  const float v01 = c0 * p01[0] + c1 * p01[1];
+  const float v10 = c0 * p10[0] + c1 * p10[1];
+  const float v11 = c0 * p11[0] + c1 * p11[1];
+  return a0 * (b0 * v00 + b1 * v01) + a1 * (b0 * v10 + b1 * v11);
+}
+
+// gather32 with 32-bit offsets: the cell's byte offset o and the half-node
+// shifts (byte offset of one step down along each axis, or 0)
+__device__ __forceinline__ void gather_o(const Grid32 &G, uint32_t o, uint32_t sl,
+                                         uint32_t sk, uint32_t hx, uint32_t hy,
+                                         uint32_t hz, const AxisW32 &X,
+                                         const AxisW32 &Y, const AxisW32 &Z,
+                                         float f[6]) {
+  const uint32_t ox = o - hx, oy = o - hy, oz = o - hz;
+  f[0] = lerp8o(G.ex, oy - hz, sl, sk, X.i0, X.i1, Y.h0, Y.h1, Z.h0, Z.h1);
+  f[1] = lerp8o(G.ey, ox - hz, sl, sk, X.h0, X.h1, Y.i0, Y.i1, Z.h0, Z.h1);
+  f[2] = lerp8o(G.ez, ox - hy, sl, sk, X.h0, X.h1, Y.h0, Y.h1, Z.i0, Z.i1);
+  f[3] = lerp8o(G.btx, ox, sl, sk, X.h0, X.h1, Y.i0, Y.i1, Z.i0, Z.i1);
+  f[4] = lerp8o(G.bty, oy, sl, sk, X.i0, X.i1, Y.h0, Y.h1, Z.i0, Z.i1);
+  f[5] = lerp8o(G.btz, oz, sl, sk, X.i0, X.i1, Y.i0, Y.i1, Z.h0, Z.h1);
+}
+
+// window of one workgroup, uniform
+struct WinS {
+  int k0, l0, m0, WY, WZ;
+  int vk0, vk1, vl0, vl1, vm0, vm1;
+  bool on;
+};
+
+__device__ inline bool in_win_s(const WinS &w, int k, int l, int m) {
+  return w.on && (unsigned)(k - w.vk0) <= (unsigned)(w.vk1 - w.vk0) &&
+         (unsigned)(l - w.vl0) <= (unsigned)(w.vl1 - w.vl0) &&
+         (unsigned)(m - w.vm0) <= (unsigned)(w.vm1 - w.vm0);
+}
+
+// J (PUSH_J) or n (DRIFT_N) deposit of one particle at cell (k,l,m),
+// offsets (ox,oy,oz)
+template <bool DET, typename A>
+__device__ __forceinline__ void deposit(const Geom &g, const WinS &w, const Walk &wk,
+                                        char *acc, bool lds_ok, bool current,
+                                        int k, int l, int m, float ox, float oy,
+                                        float oz, float vx, float vy, float vz,
+                                        const Push32 &kp, A *jx, A *jy, A *jz,
+                                        A *nd) {
+  const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+  if (lds_ok) {
+    const int sl = 8 * w.WZ, sk = 8 * w.WY * w.WZ;
+    const int c0 = (((k - w.k0) * w.WY + (l - w.l0)) * w.WZ + (m - w.m0)) * 8;
+    const int hx = X.bh != k ? sk : 0, hy = Y.bh != l ? sl : 0, hz = Z.bh != m ? 8 : 0;
+    if (current) {
+      const float qs = kp.qw * kp.fx;
+      lds_dep8(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, X.i0, X.i1, Y.h0,
+               Y.h1, Z.h0, Z.h1);
+      lds_dep8(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Y.i0,
+               Y.i1, Z.h0, Z.h1);
+      lds_dep8(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, X.h0, X.h1, Y.h0,
+               Y.h1, Z.i0, Z.i1);
+    } else {
+      lds_dep8(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, X.h0,
+               X.h1, Y.h0, Y.h1, Z.h0, Z.h1);
     }
-#else
-    lds_add(lds + i, qa0 * (b0 * c0), s);
-    lds_add(lds + i + 1, qa0 * (b0 * c1), s);
-    lds_add(lds + i + WZ, qa0 * (b1 * c0), s);
-    lds_add(lds + i + WZ + 1, qa0 * (b1 * c1), s);
-    lds_add(lds + i + WYZ, qa1 * (b0 * c0), s);
-    lds_add(lds + i + WYZ + 1, qa1 * (b0 * c1), s);
-    lds_add(lds + i + WYZ + WZ, qa1 * (b1 * c0), s);
-    lds_add(lds + i + WYZ + WZ + 1, qa1 * (b1 * c1), s);
-#endif
   } else {
-    const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
-    const int64_t i = g.idx(a, b, c);
-    glob_add(glob + i, qa0 * (b0 * c0), s);
-    glob_add(glob + i + 1, qa0 * (b0 * c1), s);
-    glob_add(glob + i + dl, qa0 * (b1 * c0), s);
-    glob_add(glob + i + dl + 1, qa0 * (b1 * c1), s);
-    glob_add(glob + i + dk, qa1 * (b0 * c0), s);
-    glob_add(glob + i + dk + 1, qa1 * (b0 * c1), s);
-    glob_add(glob + i + dk + dl, qa1 * (b1 * c0), s);
-    glob_add(glob + i + dk + dl + 1, qa1 * (b1 * c1), s);
+    const float s = DET ? kp.fx : 1.0f;
+    if (current) {
+      glob_dep8(jx, g, X.bi, Y.bh, Z.bh, kp.qw * vx * s, X.i0, X.i1, Y.h0, Y.h1,
+                Z.h0, Z.h1);
+      glob_dep8(jy, g, X.bh, Y.bi, Z.bh, kp.qw * vy * s, X.h0, X.h1, Y.i0, Y.i1,
+                Z.h0, Z.h1);
+      glob_dep8(jz, g, X.bh, Y.bh, Z.bi, kp.qw * vz * s, X.h0, X.h1, Y.h0, Y.h1,
+                Z.i0, Z.i1);
+    } else {
+      glob_dep8(nd, g, X.bh, Y.bh, Z.bh, kp.w * s, X.h0, X.h1, Y.h0, Y.h1, Z.h0,
+                Z.h1);
+    }
   }
 }
 
@@ -275,79 +393,72 @@ __device__ inline void dep8(u64 *lds, G *glob, bool in_lds, const Win &w,
 // drift or gather + Boris, half drift, J deposit at x^{n+1/2}; DRIFT_N: hot
 // loop B (:818-867) - half drift, density deposit at x^{n+1}; BOUNDARY:
 // sr_particle (:2041-2531) on one rank.
-template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, typename A>
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, typename A>
 __device__ __forceinline__ void particle_step(
-    const Grid32 &G, const Push32 &kp, const Win &w, u64 *acc, A *jx, A *jy,
-    A *jz, A *nd, int32_t &cl, float &ox, float &oy, float &oz, float &vx,
-    float &vy, float &vz) {
+    const Grid32 &G, const Push32 &kp, const WinS &w, const Walk &wk, char *acc,
+    A *jx, A *jy, A *jz, A *nd, int32_t &cl, float &ox, float &oy, float &oz,
+    float &vx, float &vy, float &vz) {
   const Geom &g = G.g;
   if (cl < 0) return;
   int k, l, m;
   unpack(cl, k, l, m);
-    const bool lds_ok = in_win(w, k, l, m);
-    if (PUSH_J) {
-      const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
-      const bool safe = k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 &&
-                        m >= 1 && m <= g.SZ - 2;
-      if (safe && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
-                    yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2])) {
-        float f[6];
-        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-        gather32(G, X, Y, Z, f);
-        boris32(kp, f, vx, vy, vz);
+  const bool lds_ok = in_win_s(w, k, l, m);
+  const auto inside = [&]() {
+    return (unsigned)(k - 1) <= (unsigned)(g.SX - 3) &&
+           (unsigned)(l - 1) <= (unsigned)(g.SY - 3) &&
+           (unsigned)(m - 1) <= (unsigned)(g.SZ - 3);
+  };
+  if (PUSH_J) {
+    const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
+    if (inside() && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
+                      yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2])) {
+      const uint32_t sl = 4u * g.SZ, sk = 4u * g.SY * g.SZ;
+      const uint32_t o = (((uint32_t)k * g.SY + l) * g.SZ + m) * 4u;
+      const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+      float f[6];
+      gather_o(G, o, sl, sk, X.bh != k ? sk : 0u, Y.bh != l ? sl : 0u,
+               Z.bh != m ? 4u : 0u, X, Y, Z, f);
+      boris32(kp, f, vx, vy, vz);
+    }
+    drift_axis(k, ox, vx, kp.hdt);
+    drift_axis(l, oy, vy, kp.hdt);
+    drift_axis(m, oz, vz, kp.hdt);
+    if (inside())
+      deposit<DET>(g, w, wk, acc, lds_ok, true, k, l, m, ox, oy, oz, vx, vy, vz,
+                   kp, jx, jy, jz, nd);
+  }
+  if (DRIFT_N) {
+    drift_axis(k, ox, vx, kp.hdt);
+    drift_axis(l, oy, vy, kp.hdt);
+    drift_axis(m, oz, vz, kp.hdt);
+    if (inside())
+      deposit<DET>(g, w, wk, acc, lds_ok, false, k, l, m, ox, oy, oz, vx, vy, vz,
+                   kp, jx, jy, jz, nd);
+  }
+  int32_t out = pack(k, l, m);
+  if (BOUNDARY) {
+    // sr_particle (:2041-2531): z must stay in [2, NZ+2) cells; x and y
+    // wrap periodically (shift by the box length) or drop
+    bool dead = !(m >= 2 && m < g.NZ + 2) || k < 0 || l < 0 || k >= g.SX ||
+                l >= g.SY;
+    if (!dead) {
+      if (k >= g.NX + 2) {
+        if (kp.periodic[0]) k -= g.NX; else dead = true;
+      } else if (k < 2) {
+        if (kp.periodic[0]) k += g.NX; else dead = true;
       }
-      drift_axis(k, ox, vx, kp.hdt);
-      drift_axis(l, oy, vy, kp.hdt);
-      drift_axis(m, oz, vz, kp.hdt);
-      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
-          m <= g.SZ - 2) {
-        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-        dep8(acc + 0 * kWin, jx, lds_ok, w, g, X.bi, Y.bh, Z.bh, X.i0,
-                  X.i1, Y.h0, Y.h1, Z.h0, Z.h1, kp.qw * vx, kp.fx);
-        dep8(acc + 1 * kWin, jy, lds_ok, w, g, X.bh, Y.bi, Z.bh, X.h0,
-                  X.h1, Y.i0, Y.i1, Z.h0, Z.h1, kp.qw * vy, kp.fx);
-        dep8(acc + 2 * kWin, jz, lds_ok, w, g, X.bh, Y.bh, Z.bi, X.h0,
-                  X.h1, Y.h0, Y.h1, Z.i0, Z.i1, kp.qw * vz, kp.fx);
+      if (l >= g.NY + 2) {
+        if (kp.periodic[1]) l -= g.NY; else dead = true;
+      } else if (l < 2) {
+        if (kp.periodic[1]) l += g.NY; else dead = true;
       }
     }
-    if (DRIFT_N) {
-      drift_axis(k, ox, vx, kp.hdt);
-      drift_axis(l, oy, vy, kp.hdt);
-      drift_axis(m, oz, vz, kp.hdt);
-      if (k >= 1 && k <= g.SX - 2 && l >= 1 && l <= g.SY - 2 && m >= 1 &&
-          m <= g.SZ - 2) {
-        const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-        dep8(acc + 3 * kWin, nd, lds_ok, w, g, X.bh, Y.bh, Z.bh, X.h0,
-                  X.h1, Y.h0, Y.h1, Z.h0, Z.h1, kp.w, kp.fx);
-      }
-    }
-    int32_t out = pack(k, l, m);
-    if (BOUNDARY) {
-      // sr_particle (:2041-2531): z must stay in [2, NZ+2) cells; x and y
-      // wrap periodically (shift by the box length) or drop
-      bool dead = !(m >= 2 && m < g.NZ + 2) || k < 0 || l < 0 || k >= g.SX ||
-                  l >= g.SY;
-      if (!dead) {
-        if (k >= g.NX + 2) {
-          if (kp.periodic[0]) k -= g.NX; else dead = true;
-        } else if (k < 2) {
-          if (kp.periodic[0]) k += g.NX; else dead = true;
-        }
-        if (l >= g.NY + 2) {
-          if (kp.periodic[1]) l -= g.NY; else dead = true;
-        } else if (l < 2) {
-          if (kp.periodic[1]) l += g.NY; else dead = true;
-        }
-      }
-      out = dead ? -1 : pack(k, l, m);
-    } else if (k < 0 || l < 0 || m < 0 || k >= g.SX || l >= g.SY || m >= g.SZ) {
-      out = -1;  // left the array: not representable
-    }
-    cl = out;
+    out = dead ? -1 : pack(k, l, m);
+  } else if (k < 0 || l < 0 || m < 0 || k >= g.SX || l >= g.SY || m >= g.SZ) {
+    out = -1;  // left the array: not representable
+  }
+  cl = out;
 }
-
-// ---------------------------------------------------------------------------
-// the fused particle pass
 
 template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET>
 __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
@@ -355,7 +466,7 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
     typename GlobT<DET>::T *jx, typename GlobT<DET>::T *jy,
     typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd) {
   using A = typename GlobT<DET>::T;
-  __shared__ u64 acc[4 * kWin];
+  __shared__ u64 acc_mem[4 * kWin];
   __shared__ Win sw;
   __shared__ int64_t s_range[2];
   const Geom &g = G.g;
@@ -382,10 +493,26 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
   __syncthreads();
   const int64_t s = s_range[0], e = s_range[1];
   if (s >= e) return;
-  const Win w = sw;
-  const int WN = w.on ? w.WX * w.WY * w.WZ : 0;
-  for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin + i % WN] = 0;
+  // the window into scalar registers
+  WinS w;
+  w.on = __builtin_amdgcn_readfirstlane(sw.on ? 1 : 0) != 0;
+  w.k0 = sw.k0;
+  w.l0 = sw.l0;
+  w.m0 = sw.m0;
+  w.WY = __builtin_amdgcn_readfirstlane(sw.WY);
+  w.WZ = __builtin_amdgcn_readfirstlane(sw.WZ);
+  w.vk0 = sw.vk0;
+  w.vk1 = sw.vk1;
+  w.vl0 = sw.vl0;
+  w.vl1 = sw.vl1;
+  w.vm0 = sw.vm0;
+  w.vm1 = sw.vm1;
+  const int WX = __builtin_amdgcn_readfirstlane(sw.WX);
+  const int WN = w.on ? WX * w.WY * w.WZ : 0;
+  for (int i = tid; i < 4 * WN; i += blockDim.x) acc_mem[(i / WN) * kWin + i % WN] = 0;
   __syncthreads();
+  char *acc = (char *)acc_mem;
+  const Walk wk = make_walk(w.WY, w.WZ);
 
   // each thread takes PIC_PPT32 consecutive particles with vector loads;
   // slots of a boundary group that belong to a neighbouring range are left
@@ -393,10 +520,12 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
   typedef int vi __attribute__((ext_vector_type(PIC_PPT32)));
   typedef float vf __attribute__((ext_vector_type(PIC_PPT32)));
   constexpr int N = PIC_PPT32;
-  constexpr int SH = N == 4 ? 2 : N == 2 ? 1 : 0;
-  const int64_t q0 = s >> SH, q1 = (e + N - 1) >> SH;
-  for (int64_t q = q0 + tid; q < q1; q += blockDim.x) {
-    const int64_t i0 = q << SH;
+  constexpr int SH = N == 8 ? 3 : N == 4 ? 2 : N == 2 ? 1 : 0;
+  // slot indices fit in int32 (the host bounds the stored count)
+  const int s32 = (int)s, e32 = (int)e;
+  const int q0 = s32 >> SH, q1 = (e32 + N - 1) >> SH;
+  for (int q = q0 + tid; q < q1; q += blockDim.x) {
+    const int i0 = q << SH;
     vi cs = *(const vi *)(P.cell + i0);
     vf xs = *(const vf *)(P.ox + i0);
     vf ys = *(const vf *)(P.oy + i0);
@@ -404,15 +533,15 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
     vf us = *(const vf *)(P.vx + i0);
     vf vs = *(const vf *)(P.vy + i0);
     vf ws = *(const vf *)(P.vz + i0);
-    const bool full = i0 >= s && i0 + N - 1 < e;
+    const bool full = i0 >= s32 && i0 + N - 1 < e32;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      const int64_t i = i0 + j;
-      if (i < s || i >= e) continue;
+      const int i = i0 + j;
+      if (i < s32 || i >= e32) continue;
       int c = cs[j];
       float x = xs[j], y = ys[j], z = zs[j], u = us[j], v = vs[j], ww = ws[j];
-      particle_step<PUSH_J, DRIFT_N, BOUNDARY>(G, kp, w, acc, jx, jy, jz, nd, c, x,
-                                               y, z, u, v, ww);
+      particle_step<PUSH_J, DRIFT_N, BOUNDARY, DET>(G, kp, w, wk, acc, jx, jy, jz,
+                                                    nd, c, x, y, z, u, v, ww);
       cs[j] = c;
       xs[j] = x;
       ys[j] = y;
@@ -432,8 +561,8 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const int64_t i = i0 + j;
-        if (i < s || i >= e) continue;
+        const int i = i0 + j;
+        if (i < s32 || i >= e32) continue;
         P.cell[i] = cs[j];
         P.ox[i] = xs[j];
         P.oy[i] = ys[j];
@@ -453,7 +582,7 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
   for (int c = 0; c < 4; ++c) {
     A *dst = c == 0 ? jx : c == 1 ? jy : c == 2 ? jz : nd;
     for (int j = tid; j < WN; j += blockDim.x) {
-      const u64 v = acc[c * kWin + j];
+      const u64 v = acc_mem[c * kWin + j];
       if (v == 0) continue;
       const int kk = w.k0 + j / WYZ, ll = w.l0 + (j / w.WZ) % w.WY,
                 mm = w.m0 + j % w.WZ;
@@ -759,27 +888,22 @@ __global__ void k_gather32(Particles32 dst, Particles32 src,
 
 __global__ __launch_bounds__(256) void k_sort_keys32(
     Geom g, TileGeom tg, const int32_t *__restrict__ cell, int64_t n,
-    uint32_t *keys, uint32_t *tile_count) {
+    uint32_t *keys) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
-  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
   uint32_t key = ncell;
-  if (i < n) {
-    const int32_t c = cell[i];
-    if (c >= 0) {
-      int k, l, m;
-      unpack(c, k, l, m);
-      k -= 2;
-      l -= 2;
-      m -= 2;
-      if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
-        key = tg.key(k, l, m);
-    }
-    keys[i] = key;
+  const int32_t c = cell[i];
+  if (c >= 0) {
+    int k, l, m;
+    unpack(c, k, l, m);
+    k -= 2;
+    l -= 2;
+    m -= 2;
+    if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
+      key = tg.key(k, l, m);
   }
-  // per-tile counts, one atomic per run of equal tiles in a wave
-  const bool v = i < n && key < ncell;
-  count_tile_runs(v, v ? key / cpt : 0u, tile_count);
+  keys[i] = key;
 }
 
 __global__ __launch_bounds__(256) void k_count_live32(
@@ -989,8 +1113,8 @@ int init_state(pic_ctx *ctx) {
   if (e != hipSuccess)
     return set_error(ctx, PIC_E_HIP, "fp32 sigma: %s", hipGetErrorString(e));
   {
-    // fixed-point unit: 2^-40 of the largest single contribution q w c/dx^3
-    // (per-node sums of a few thousand contributions stay far below 2^63)
+    // fixed-point unit: 2^-30 of the largest single contribution q w c/dx^3
+    // (per-node sums of millions of contributions stay far below 2^63)
     const pic_config &c = ctx->cfg;
     const double wmax = c.weight[0] > c.weight[1] ? c.weight[0] : c.weight[1];
     const double qmax = c.q > 1.0 ? c.q : 1.0;
@@ -1154,6 +1278,10 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
          int do_boundary) {
   const int64_t n = ctx->n[sp];
   if (n == 0) return 0;
+  if (n > (int64_t)INT32_MAX - 8)
+    return set_error(ctx, PIC_E_CAPACITY,
+                     "fp32 particle pass: %lld stored particles exceed the "
+                     "32-bit slot index", (long long)n);
   State32 *S = st(ctx);
   const TileGeom tg = tile_geom(ctx);
   TileRun tr;
@@ -1281,19 +1409,18 @@ int sort(pic_ctx *ctx, int sp) {
     return 0;
   }
   Particles32 src = p32(ctx, sp, ctx->cur[sp]);
-  PIC_HIP(ctx, hipMemsetAsync(ctx->tile_count, 0, (size_t)nt * sizeof(uint32_t),
-                              ctx->stream));
   k_sort_keys32<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
-      g, tg, src.cell, n, ctx->keys, ctx->tile_count);
+      g, tg, src.cell, n, ctx->keys);
   PIC_LAUNCH_CHECK(ctx);
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= ncell) ++bits;
   int rc = sort_keys(ctx, ctx->keys, ctx->keys_alt, ctx->idx, n, bits);
   if (rc) return rc;
-  rc = scan_tiles(ctx, sp);
+  rc = tile_bounds(ctx, ctx->keys_alt, n, (uint32_t)(tg.TX * tg.TY * tg.TZ), nt,
+                   ctx->tile_start[sp]);
   if (rc) return rc;
-  // the valid count is the total of the tile counts
+  // the kept count is the start of the drop tile
   PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->tile_start[sp] + nt,
                               sizeof(uint32_t), hipMemcpyDeviceToHost,
                               ctx->stream));

@@ -746,48 +746,25 @@ __global__ void k_gather_particles(Particles64 dst, Particles64 src,
 // sorting key (:7488-7497): ((int)(x/dx)-2, (int)(y/dx)-2, (int)(z/dx)-2),
 // tile-major (the reference's lexicographic order with m fastest when one
 // tile spans the interior); particles outside the interior (and dead ones)
-// get the key ncell and are dropped after the sort.  Grid-stride loop with a
-// bounded grid so the valid counter sees one atomic per workgroup.
+// get the key ncell and are dropped after the sort.
 __global__ __launch_bounds__(256) void k_sort_keys(Geom g, TileGeom tg,
                                                    Particles64 P, int64_t n,
-                                                   double dx, uint32_t *keys,
-                                                   int64_t *valid,
-                                                   uint32_t *tile_count) {
+                                                   double dx, uint32_t *keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
-  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
-  unsigned int cnt = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t nround = (n + stride - 1) / stride * stride;
-  // whole waves iterate together and reach the per-wave tile counting from
-  // one call site, so its ballots see every lane
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nround;
-       i += stride) {
-    uint32_t key = ncell;
-    if (i < n) {
-      const double x = P.x[i], y = P.y[i], z = P.z[i];
-      const double xd = x / dx, yd = y / dx, zd = z / dx;
-      // range test before the conversions so that (int) never sees a value
-      // outside int range or NaN
-      if (xd > -1.0 && xd < g.NX + 3 && yd > -1.0 && yd < g.NY + 3 &&
-          zd > -1.0 && zd < g.NZ + 3) {
-        const int k = (int)(xd) - 2, l = (int)(yd) - 2, m = (int)(zd) - 2;
-        if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
-          key = tg.key(k, l, m);
-      }
-      keys[i] = key;
-      cnt += key < ncell;
-    }
-    count_tile_runs(key < ncell, key < ncell ? key / cpt : 0u, tile_count);
+  uint32_t key = ncell;
+  const double x = P.x[i], y = P.y[i], z = P.z[i];
+  const double xd = x / dx, yd = y / dx, zd = z / dx;
+  // range test before the conversions so that (int) never sees a value
+  // outside int range or NaN
+  if (xd > -1.0 && xd < g.NX + 3 && yd > -1.0 && yd < g.NY + 3 &&
+      zd > -1.0 && zd < g.NZ + 3) {
+    const int k = (int)(xd) - 2, l = (int)(yd) - 2, m = (int)(zd) - 2;
+    if (k >= 0 && k < g.NX && l >= 0 && l < g.NY && m >= 0 && m < g.NZ)
+      key = tg.key(k, l, m);
   }
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
-  __shared__ unsigned int part[kBlock / 64];
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
-    if (t) atomicAdd((unsigned long long *)valid, t);
-  }
+  keys[i] = key;
 }
 
 __global__ __launch_bounds__(256) void k_count_live(
@@ -1124,25 +1101,23 @@ int sort(pic_ctx *ctx, int sp) {
   Particles64 src = p64(ctx, sp, ctx->cur[sp]);
   const TileGeom tg = tile_geom(ctx);
   const int nt = tg.ntiles();
-  PIC_HIP(ctx, hipMemsetAsync(ctx->dev_counter, 0, sizeof(int64_t), ctx->stream));
-  PIC_HIP(ctx, hipMemsetAsync(ctx->tile_count, 0, (size_t)nt * sizeof(uint32_t),
-                              ctx->stream));
-  k_sort_keys<<<(unsigned)(div_up(n, kBlock) < 4096 ? div_up(n, kBlock) : 4096),
-                kBlock, 0, ctx->stream>>>(g, tg, src, n, ctx->cfg.dx, ctx->keys,
-                                          ctx->dev_counter, ctx->tile_count);
+  k_sort_keys<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
+      g, tg, src, n, ctx->cfg.dx, ctx->keys);
   PIC_LAUNCH_CHECK(ctx);
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= ncell) ++bits;
   int rc = sort_keys(ctx, ctx->keys, ctx->keys_alt, ctx->idx, n, bits);
   if (rc) return rc;
-  rc = scan_tiles(ctx, sp);
+  rc = tile_bounds(ctx, ctx->keys_alt, n, (uint32_t)(tg.TX * tg.TY * tg.TZ), nt,
+                   ctx->tile_start[sp]);
   if (rc) return rc;
-  PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->dev_counter,
-                              sizeof(int64_t), hipMemcpyDeviceToHost,
+  // the kept count is the start of the drop tile
+  PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, ctx->tile_start[sp] + nt,
+                              sizeof(uint32_t), hipMemcpyDeviceToHost,
                               ctx->stream));
   PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int64_t valid = ctx->host_counter[0];
+  const int64_t valid = (int64_t)(*(uint32_t *)ctx->host_counter);
   Particles64 dst = p64(ctx, sp, 1 - ctx->cur[sp]);
   if (valid > 0)
     k_gather_particles<<<(unsigned)div_up(valid, kBlock), kBlock, 0,

@@ -106,7 +106,6 @@ struct pic_ctx {
 
   // sort tiles: per-species particle ranges of each tile after the last
   // sort; [0, sorted_end) is tile-sorted, later slots are the inflow tail
-  uint32_t *tile_count = nullptr;                 // ntiles
   uint32_t *tile_start[2] = {nullptr, nullptr};   // ntiles + 1 per species
   int64_t sorted_end[2] = {0, 0};
 
@@ -248,7 +247,9 @@ int select_flagged(pic_ctx *ctx, const uint8_t *flags, uint32_t *idx_out,
 int sort_keys(pic_ctx *ctx, uint32_t *keys_in, uint32_t *keys_out,
               uint32_t *idx_out, int64_t n, int end_bit);
 int ensure_prim_tmp(pic_ctx *ctx, size_t bytes);
-// exclusive scan of ctx->tile_count into ctx->tile_start[sp] (nt + 1 values)
-int scan_tiles(pic_ctx *ctx, int sp);
+// tile_start[0..nt] from keys sorted ascending (cpt cells per tile; keys
+// >= nt * cpt are dropped slots and land in tile nt)
+int tile_bounds(pic_ctx *ctx, const uint32_t *skeys, int64_t n, uint32_t cpt,
+                int nt, uint32_t *tile_start);
 
 }  // namespace pic

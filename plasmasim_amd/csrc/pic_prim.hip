@@ -57,35 +57,27 @@ int sort_keys(pic_ctx *ctx, uint32_t *keys_in, uint32_t *keys_out,
   return 0;
 }
 
-// exclusive scan of the per-tile counts in one workgroup: start[t], t<=nt
-__global__ __launch_bounds__(1024) void k_scan_tiles(const uint32_t *cnt,
-                                                     uint32_t *start, int nt) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (nt + 1023) / 1024;
-  const int lo = t * per, hi = min(nt, lo + per);
-  uint32_t s = 0;
-  for (int i = lo; i < hi; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[t] - s;
-  for (int i = lo; i < hi; ++i) {
-    start[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) start[nt] = part[1023];
+// tile_start[t] = first sorted slot of tile t, from keys sorted ascending.
+// Keys of dropped slots are at least nt * cpt and map to tile nt, so
+// tile_start[nt] is the number of kept particles.  Each slot writes the
+// starts of the tiles that begin at it (empty tiles included), so every
+// entry is written exactly once and no counting pass is needed.
+__global__ void k_tile_bounds(const uint32_t *__restrict__ skeys, int64_t n,
+                              uint32_t cpt, int nt, uint32_t *tile_start) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int tc = (int)min((uint32_t)nt, skeys[i] / cpt);
+  const int tp = i == 0 ? -1 : (int)min((uint32_t)nt, skeys[i - 1] / cpt);
+  for (int t = tp + 1; t <= tc; ++t) tile_start[t] = (uint32_t)i;
+  if (i == n - 1)
+    for (int t = tc + 1; t <= nt; ++t) tile_start[t] = (uint32_t)n;
 }
 
-int scan_tiles(pic_ctx *ctx, int sp) {
-  const int nt = tile_geom(ctx).ntiles();
-  k_scan_tiles<<<1, 1024, 0, ctx->stream>>>(ctx->tile_count,
-                                            ctx->tile_start[sp], nt);
+int tile_bounds(pic_ctx *ctx, const uint32_t *skeys, int64_t n, uint32_t cpt,
+                int nt, uint32_t *tile_start) {
+  if (n == 0) return 0;
+  k_tile_bounds<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0,
+                  ctx->stream>>>(skeys, n, cpt, nt, tile_start);
   PIC_LAUNCH_CHECK(ctx);
   return 0;
 }

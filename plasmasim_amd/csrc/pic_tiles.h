@@ -99,23 +99,4 @@ __device__ inline bool in_win(const Win &w, int k, int l, int m) {
          m >= w.vm0 && m <= w.vm1;
 }
 
-// run-aggregated per-tile counting over one wave: lanes of a run of equal
-// tiles add once (particles are nearly sorted, so runs are long)
-__device__ inline void count_tile_runs(bool v, uint32_t tile,
-                                       uint32_t *tile_count) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t t = v ? tile : 0xffffffffu;
-  const uint32_t prev = __shfl_up(t, 1);
-  const bool head = v && (lane == 0 || prev != t);
-  const uint64_t heads = __ballot(head);
-  const uint64_t valid_mask = __ballot(v);
-  if (head) {
-    const uint64_t after = lane == 63 ? 0 : (heads >> (lane + 1));
-    const int next = after ? lane + 1 + __ffsll((long long)after) - 1 : 64;
-    const uint64_t span = (next >= 64 ? ~0ull : ((1ull << next) - 1)) &
-                          ~((1ull << lane) - 1);
-    atomicAdd(tile_count + t, (unsigned)__popcll(valid_mask & span));
-  }
-}
-
 }  // namespace pic
