@@ -17,6 +17,10 @@ against the FP64-forward adjoint and central differences.
 species (838,860,800 particles), line-current dipole, DST Poisson
 correction every step, inflow every 250 steps, tile sort every 20.
 
+--workload checkpoint: pic_checkpoint_save and pic_checkpoint_load of the
+default workload's state after a few steps (FP32 and FP64 forms), in bytes
+per second of checkpoint file, to a file under $TMPDIR (or /tmp).
+
 Prints one JSON line: value = particle-steps/s of the chosen form (FP32 by
 default), with the plain FP64 form's rate, the roofline of the dominant
 kernel (the fused particle pass) and a CPU baseline (the C oracle on the
@@ -223,13 +227,56 @@ def run_adjoint(ps, steps=200, ckpt=20, radius=10.0, nquad=1024,
     return out
 
 
+def run_checkpoint(ps, steps=3):
+    """Save and load times of the default workload's state in both forms."""
+    import tempfile
+    out = {}
+    d = os.environ.get("TMPDIR", "/tmp")
+    for form, prec in (("fp32", ps.PIC_FP32), ("fp64", ps.PIC_FP64)):
+        c = ps.preset(2)
+        c.precision = prec
+        s = ps.Simulation(c)
+        s.load_particles()
+        s.begin()
+        s.step(steps)
+        s.sync()
+        fd, path = tempfile.mkstemp(suffix=".ckpt", dir=d)
+        os.close(fd)
+        try:
+            t0 = time.perf_counter()
+            s.checkpoint_save(path)
+            t_save = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            e0 = s.diagnose()
+            s.close()
+            r = ps.Simulation(c)
+            t0 = time.perf_counter()
+            r.checkpoint_load(path)
+            r.sync()
+            t_load = time.perf_counter() - t0
+            e1 = r.diagnose()
+            r.close()
+        finally:
+            os.remove(path)
+        same = all(e0[k] == e1[k] for k in ("field_energy_e", "field_energy_b",
+                                            "charge")) and \
+            list(e0["count"]) == list(e1["count"])
+        out[form] = {"bytes": size, "save_s": t_save, "load_s": t_load,
+                     "save_GBps": size / t_save / 1e9, "load_GBps": size / t_load / 1e9,
+                     "diagnostics_identical": bool(same)}
+        print(f"{form}: {size / 1e9:.2f} GB, save {t_save:.2f} s, load {t_load:.2f} s",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--workload", choices=["3d", "2d3v", "adjoint"], default="3d")
+    ap.add_argument("--workload", choices=["3d", "2d3v", "adjoint", "checkpoint"],
+                    default="3d")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -256,6 +303,18 @@ def main():
                                    "checkpoint every 20, stress flux through r = 10 cells, "
                                    "1024 quadrature points"},
             "adjoint": r}))
+        return
+    if a.workload == "checkpoint":
+        r = run_checkpoint(ps)
+        print(json.dumps({
+            "metric": "checkpoint save bytes/s on 1 MI355X (128^3 x 64 ppc state)",
+            "value": r["fp32"]["save_GBps"] * 1e9, "unit": "B/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 state (fp64 alongside)",
+            "data": "synthetic seeded plasma after 3 steps",
+            "config": {"workload": "checkpoint of the 3D state, 128^3 cells x 64 ppc "
+                                   "x 2 species, file under $TMPDIR"},
+            "checkpoint": r}))
         return
     main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload)
     other = None

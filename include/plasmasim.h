@@ -42,6 +42,8 @@ extern "C" {
 #define PIC_E_HIP (-3)      /* HIP runtime error */
 #define PIC_E_CAPACITY (-4) /* particle store would overflow its capacity */
 #define PIC_E_STATE (-5)    /* call not valid in the current state */
+#define PIC_E_IO (-6)       /* file could not be opened, read or written */
+#define PIC_E_FORMAT (-7)   /* checkpoint header, config or checksum mismatch */
 
 /* numeric forms */
 #define PIC_FP64 0 /* plain form: reference FP64 arithmetic */
@@ -212,9 +214,12 @@ int pic_step(pic_ctx *ctx, int nsteps);
 /* The reference's pre-loop inflow (:673-674) followed by nothing else;
  * call once after pic_load_particles and before the first pic_step. */
 int pic_begin(pic_ctx *ctx);
-/* Step counter (c) and cadence counters (c_add, c_thrust). */
+/* Step counter (c) and cadence counters (c_add, c_thrust).  pic_step keeps
+ * them; a host loop that calls the routines itself keeps its own (fdtd's
+ * locals, :630-686) and sets them before pic_checkpoint_save. */
 int pic_get_counters(const pic_ctx *ctx, int64_t *step, int32_t *c_add,
                      int32_t *c_sort);
+int pic_set_counters(pic_ctx *ctx, int64_t step, int32_t c_add, int32_t c_sort);
 
 /* ---- kernel timing (HIP events on the context stream) ---- */
 #define PIC_T_PUSH 0     /* fused particle pass */
@@ -250,6 +255,60 @@ int pic_timing_reset(pic_ctx *ctx);
 int pic_adj_forward(pic_ctx *ctx, const double params[4], int nsteps,
                     int ckpt_every, double radius, int nquad, double *loss);
 int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]);
+
+/* ---- checkpoint and resume ----
+ *
+ * Replaces the reference's suspend_job / continue_job (:7629-7911), which
+ * dump the raw Grid records and Particle structs (pointers included) and
+ * re-seed the random streams on resume.  A checkpoint holds the complete
+ * state of a context between steps in its own numeric form: the 17 field
+ * planes as stored (FP32 form: B - B0 in the B planes, plus B0 in double),
+ * the particle slots of both species (tombstones included, in slot order),
+ * the tile ranges of the last sort, the step counters and the Philox event
+ * counter of the inflow, so a resumed run continues exactly as the original
+ * (bit-identically in the FP32 deterministic mode; to atomic summation
+ * order otherwise).  FP64-form checkpoints are also read and written by the
+ * CPU oracle.
+ *
+ * File layout (little-endian): pic_checkpoint_header (256 bytes), then the
+ * payload: the pic_config (config_bytes), 17 planes of (nx+4)(ny+4)(nz+4)
+ * values in the order of enum pic_field (double, or float in the FP32
+ * form), FP32 form only: 3 double planes of B0; then per species either six
+ * double arrays x, y, z, vx, vy, vz (FP64) or an int32 cell array and six
+ * float arrays ox, oy, oz, vx, vy, vz (FP32) of n[s] values; then per
+ * species ntiles + 1 uint32 tile starts (none if ntiles is 0).  checksum is
+ * a 64-bit hash of the payload: four interleaved FNV-1a-style lanes
+ * h_j = (h_j ^ w) * 0x100000001b3 over the little-endian 64-bit words w of
+ * the payload (word i to lane i mod 4, the last partial word zero-padded),
+ * lanes seeded 0xcbf29ce484222325 + j and combined as
+ * h = ((h_0 * 31 + h_1) * 31 + h_2) * 31 + h_3.
+ *
+ * pic_checkpoint_load needs a context created with the same config (the
+ * capacity and the deposit mode may differ) holding at least n[s] slots.
+ * If the file's tile geometry differs from the context's, the particles are
+ * treated as unsorted until the next sort.  On PIC_E_FORMAT from a checksum
+ * mismatch the state has been overwritten: load again or recreate. */
+typedef struct pic_checkpoint_header {
+  char magic[8];           /* "PLASCKPT" */
+  uint32_t version;        /* 1 */
+  uint32_t header_bytes;   /* 256 */
+  uint32_t config_bytes;   /* sizeof(pic_config) of the writer */
+  int32_t precision;       /* PIC_FP64 or PIC_FP32 */
+  int32_t nx, ny, nz;
+  int32_t tile[3];         /* tile geometry of the tile starts */
+  int64_t step;            /* steps done (reference c) */
+  int32_t c_add, c_sort;   /* inflow and sort cadence counters */
+  uint32_t inflow_event;   /* Philox event id of the next inflow */
+  uint32_t ntiles;         /* 0: no tile starts stored */
+  int64_t n[2];            /* stored slots per species */
+  int64_t sorted_end[2];   /* tile-sorted prefix per species */
+  uint64_t payload_bytes;
+  uint64_t checksum;
+  uint8_t reserved[136];
+} pic_checkpoint_header;
+
+int pic_checkpoint_save(pic_ctx *ctx, const char *path);
+int pic_checkpoint_load(pic_ctx *ctx, const char *path);
 
 #ifdef __cplusplus
 }

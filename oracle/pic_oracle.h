@@ -73,6 +73,11 @@ int orc_adj_gradient(orc_ctx *o, const double p[4], int nsteps, double R,
 double orc_adj_force(orc_ctx *o, double R, int nq);
 /* dot-product checks of the transposed field operators */
 int orc_adj_selftest(orc_ctx *o, uint64_t seed, double out[4]);
+
+/* FP64-form checkpoints in the format of include/plasmasim.h
+ * (pic_checkpoint_save / pic_checkpoint_load); pic_oracle_ckpt.c */
+int orc_checkpoint_save(orc_ctx *o, const char *path);
+int orc_checkpoint_load(orc_ctx *o, const char *path);
 int orc_adj_boris_check(const pic_config *cfg, double m, const double f[6],
                         const double v[3], const double lvp[3], double out[2]);
 

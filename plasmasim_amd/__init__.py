@@ -7,6 +7,7 @@ that interface: one method per reference routine of fdtd()
 order, errors raised as exceptions.  It performs no computation of its own.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -191,6 +192,22 @@ class Simulation:
         g = np.zeros(4)
         self._check(self._L.pic_adj_backward(self._ctx, float(dloss), _lib.dptr(g)))
         return g
+
+    def counters(self):
+        """(steps done, inflow cadence counter, sort cadence counter)."""
+        st, ca, cs = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._L.pic_get_counters(self._ctx, ctypes.byref(st),
+                                             ctypes.byref(ca), ctypes.byref(cs)))
+        return st.value, ca.value, cs.value
+
+    # -- checkpoint and resume ---------------------------------------------
+    def checkpoint_save(self, path):
+        """Write the complete state between steps (pic_checkpoint_save)."""
+        self._check(self._L.pic_checkpoint_save(self._ctx, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        """Resume from a checkpoint written with the same config."""
+        self._check(self._L.pic_checkpoint_load(self._ctx, os.fsencode(path)))
 
     # -- timing -------------------------------------------------------------
     def timing(self, on=True):

@@ -164,12 +164,15 @@ SIGNATURES = [
     ("pic_begin", ctypes.c_int, [_P]),
     ("pic_get_counters", ctypes.c_int,
      [_P, _I64P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    ("pic_set_counters", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),
     ("pic_timing_enable", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_timing_get", ctypes.c_int, [_P, ctypes.c_int, _D, _I64P]),
     ("pic_timing_reset", ctypes.c_int, [_P]),
     ("pic_adj_forward", ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_int, _D]),
     ("pic_adj_backward", ctypes.c_int, [_P, ctypes.c_double, _D]),
+    ("pic_checkpoint_save", ctypes.c_int, [_P, ctypes.c_char_p]),
+    ("pic_checkpoint_load", ctypes.c_int, [_P, ctypes.c_char_p]),
     # include/plasmasim2d.h
     ("pic2d_sizeof_config", ctypes.c_size_t, []),
     ("pic2d_sizeof_diag", ctypes.c_size_t, []),

@@ -750,6 +750,15 @@ extern "C" int pic_get_counters(const pic_ctx *ctx, int64_t *step,
   return 0;
 }
 
+extern "C" int pic_set_counters(pic_ctx *ctx, int64_t step, int32_t c_add,
+                                int32_t c_sort) {
+  if (!ctx || step < 0 || c_add < 0 || c_sort < 0) return PIC_E_ARG;
+  ctx->step = step;
+  ctx->c_add = c_add;
+  ctx->c_sort = c_sort;
+  return 0;
+}
+
 namespace pic {
 int push_fused(pic_ctx *ctx, int sp) {
   timing_begin(ctx);

@@ -1078,6 +1078,11 @@ static Grid32 grid32(pic_ctx *ctx) {
   return G;
 }
 
+int state_restored(pic_ctx *ctx) {
+  st(ctx)->fx_dirty = false;
+  return refresh_total_b(ctx);
+}
+
 int refresh_total_b(pic_ctx *ctx) {
   Fields<float> F = fields<float>(ctx);
   const Geom &g = ctx->geom;

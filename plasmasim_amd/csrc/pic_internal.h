@@ -226,6 +226,9 @@ int upload_particles(pic_ctx *, int sp, int64_t n, const double *const *h);
 int download_particles(pic_ctx *, int sp, int64_t cap, int64_t *n,
                        double *const *h);
 int finish_deposit(pic_ctx *);
+// after a checkpoint load: total-B planes rebuilt, no pending fixed-point
+// deposit
+int state_restored(pic_ctx *);
 int init_state(pic_ctx *);
 void free_state(pic_ctx *);
 int refresh_total_b(pic_ctx *);

@@ -10,9 +10,17 @@
  * (thrust_f, every 500 steps in the reference; here every --diag steps).
  *
  * Usage: pic_run [--preset N] [--steps K] [--fp64|--fp32] [--det]
- *                [--diag D] [--fused]
+ *                [--diag D] [--fused] [--suspend FILE] [--suspend-every S]
+ *                [--resume FILE]
  * --fused calls pic_step() (loops A and B fused into one particle pass)
  * instead of the routine-by-routine loop.
+ *
+ * Suspend and resume (main_func :270-283, fdtd :918-931): with --resume
+ * FILE the run continues from FILE when it exists (instead of loading the
+ * plasma), up to --steps steps in total; with --suspend FILE the state is
+ * written every --suspend-every steps and at the end.  As in the
+ * reference, a file named "finish" in the working directory stops the run
+ * at the next diagnostic step (after writing the suspend file).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -69,9 +77,19 @@ static int step_routines(pic_ctx *ctx, const pic_config *c, int *c_sort,
   return 0;
 }
 
+/* suspend_job (:7629-7675): the counters of the routine loop go with the
+ * state */
+static int suspend(pic_ctx *ctx, const char *path, int fused, int64_t c,
+                   int c_add, int c_sort) {
+  int rc;
+  if (!fused && (rc = pic_set_counters(ctx, c, c_add, c_sort))) return rc;
+  return pic_checkpoint_save(ctx, path);
+}
+
 int main(int argc, char **argv) {
   int preset = 1, steps = 100, diag = 50, fused = 0;
-  int precision = -1, det = 0;
+  int precision = -1, det = 0, suspend_every = 0;
+  const char *suspend_path = NULL, *resume_path = NULL;
   for (int i = 1; i < argc; ++i) {
     if (!strcmp(argv[i], "--preset") && i + 1 < argc) preset = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--steps") && i + 1 < argc) steps = atoi(argv[++i]);
@@ -80,10 +98,15 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[i], "--fp32")) precision = PIC_FP32;
     else if (!strcmp(argv[i], "--det")) det = 1;
     else if (!strcmp(argv[i], "--fused")) fused = 1;
+    else if (!strcmp(argv[i], "--suspend") && i + 1 < argc) suspend_path = argv[++i];
+    else if (!strcmp(argv[i], "--suspend-every") && i + 1 < argc)
+      suspend_every = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--resume") && i + 1 < argc) resume_path = argv[++i];
     else {
       fprintf(stderr,
               "usage: %s [--preset N] [--steps K] [--fp64|--fp32] [--det] "
-              "[--diag D] [--fused]\n", argv[0]);
+              "[--diag D] [--fused] [--suspend FILE] [--suspend-every S] "
+              "[--resume FILE]\n", argv[0]);
       return 2;
     }
   }
@@ -93,31 +116,53 @@ int main(int argc, char **argv) {
   if (precision >= 0) cfg.precision = precision;
   if (det) cfg.deposit = PIC_DEPOSIT_DETERMINISTIC;
   CHECK(ctx, pic_create(&cfg, &ctx));                 /* init_grid :291 */
-  CHECK(ctx, pic_load_particles(ctx));                /* :412-625 */
-  CHECK(ctx, pic_begin(ctx));                         /* :673-674 */
+  int c_sort = 0, c_add = 0;
+  int64_t c0 = 0;
+  FILE *probe = resume_path ? fopen(resume_path, "rb") : NULL;
+  if (probe) {                                        /* continue_job :282 */
+    fclose(probe);
+    CHECK(ctx, pic_checkpoint_load(ctx, resume_path));
+    CHECK(ctx, pic_get_counters(ctx, &c0, &c_add, &c_sort));
+    fprintf(stderr, "continue_job from %s at step %lld\n", resume_path,
+            (long long)c0);
+  } else {
+    CHECK(ctx, pic_load_particles(ctx));              /* :412-625 */
+    CHECK(ctx, pic_begin(ctx));                       /* :673-674 */
+  }
   fprintf(stderr, "grid %dx%dx%d ppc %d form %s deposit %s\n", cfg.nx, cfg.ny,
           cfg.nz, cfg.ppc, cfg.precision == PIC_FP64 ? "fp64" : "fp32",
           cfg.deposit == PIC_DEPOSIT_DETERMINISTIC ? "deterministic" : "atomic");
-  int c_sort = 0, c_add = 0;
   const double t0 = now();
-  for (int c = 0; c < steps; ++c) {
+  for (int64_t c = c0; c < steps; ++c) {
     if (fused) {
       CHECK(ctx, pic_step(ctx, 1));
     } else {
       CHECK(ctx, step_routines(ctx, &cfg, &c_sort, &c_add));
     }
-    if (diag > 0 && ((c + 1) % diag == 0 || c + 1 == steps)) {
+    const int last = c + 1 == steps;
+    if (suspend_path && ((suspend_every > 0 && (c + 1) % suspend_every == 0) || last))
+      CHECK(ctx, suspend(ctx, suspend_path, fused, c + 1, c_add, c_sort));
+    if (diag > 0 && ((c + 1) % diag == 0 || last)) {
       pic_diag d;
       CHECK(ctx, pic_diagnose(ctx, &d));
       const double W = d.field_energy_e + d.field_energy_b1 + d.kinetic[0] +
                        d.kinetic[1];
-      printf("%d %.3f s  Ee %.6e  Eb1 %.6e  Ki %.6e  Ke %.6e  total %.6e  "
+      printf("%lld %.3f s  Ee %.6e  Eb1 %.6e  Ki %.6e  Ke %.6e  total %.6e  "
              "Fx %.4e Fy %.4e Fz %.4e  gauss_l2 %.4e  n %lld %lld\n",
-             c + 1, now() - t0, d.field_energy_e, d.field_energy_b1,
+             (long long)(c + 1), now() - t0, d.field_energy_e, d.field_energy_b1,
              d.kinetic[0], d.kinetic[1], W, d.force[0], d.force[1],
              d.force[2], d.gauss_l2, (long long)d.count[0],
              (long long)d.count[1]);
       fflush(stdout);
+      FILE *fin = fopen("finish", "r");               /* :918-923 */
+      if (fin) {
+        fclose(fin);
+        if (suspend_path && !last)
+          CHECK(ctx, suspend(ctx, suspend_path, fused, c + 1, c_add, c_sort));
+        fprintf(stderr, "finish file: stopping after step %lld\n",
+                (long long)(c + 1));
+        break;
+      }
     }
   }
   pic_destroy(ctx);

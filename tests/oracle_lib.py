@@ -56,6 +56,8 @@ _SIG = {
     "orc_adj_gradient": (ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         _D, _D]),
     "orc_adj_force": (ctypes.c_double, [_P, ctypes.c_double, ctypes.c_int]),
+    "orc_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "orc_checkpoint_load": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "orc_adj_selftest": (ctypes.c_int, [_P, ctypes.c_uint64, _D]),
     "orc_adj_boris_check": (ctypes.c_int, [ctypes.POINTER(PicConfig), ctypes.c_double,
                                            _D, _D, _D, _D]),
@@ -206,6 +208,12 @@ class Oracle:
         self._check(self._L.orc_adj_gradient(self._o, dptr(p), nsteps, R, nq,
                                              ctypes.byref(loss), dptr(g)))
         return loss.value, g
+
+    def checkpoint_save(self, path):
+        return self._L.orc_checkpoint_save(self._o, os.fsencode(path))
+
+    def checkpoint_load(self, path):
+        return self._L.orc_checkpoint_load(self._o, os.fsencode(path))
 
     def adj_force(self, R, nq):
         return self._L.orc_adj_force(self._o, R, nq)

@@ -1,0 +1,158 @@
+"""Checkpoint and resume through the C-ABI on the GPU.
+
+- A checkpoint of the FP64 form resumed by the CPU oracle, and an oracle
+  checkpoint resumed by the library, continue like the original run
+  (relative L2 <= 1e-12 on fields and phase space, counts exact).
+- Resumed in a fresh context, a run continues like the original: to
+  summation order in the FP64 form, bit-identically in the FP32
+  deterministic mode (across sort and inflow cadences).
+- Files of the other numeric form, of another config, or with more
+  particles than the capacity are rejected."""
+import numpy as np
+import pytest
+
+import plasmasim_amd as ps
+from helpers import rel_l2
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+FIELDS = ["ex", "ey", "ez", "bx", "by", "bz", "jix", "jiy", "jiz", "jex", "jey",
+          "jez", "ni", "ne"]
+
+
+def cfg(precision=ps.PIC_FP64, preset=0, **over):
+    c = ps.preset(preset)
+    c.precision = precision
+    c.inflow_every = 3
+    c.sort_every = 4
+    for k, v in over.items():
+        setattr(c, k, v)
+    return c
+
+
+def compare(a, b, tol):
+    for n in FIELDS:
+        e = rel_l2(a.download_field(n), b.download_field(n))
+        assert e <= tol, (n, e)
+    for sp in (0, 1):
+        assert a.particle_count(sp) == b.particle_count(sp)
+        pa, pb = a.download_particles(sp), b.download_particles(sp)
+        live = ~np.isnan(pb[:, 2])
+        assert np.array_equal(np.isnan(pa[:, 2]), ~live)
+        assert rel_l2(pa[live], pb[live]) <= tol, sp
+
+
+def started(c, steps):
+    s = ps.Simulation(c)
+    s.load_particles()
+    s.begin()
+    s.step(steps)
+    return s
+
+
+def test_library_checkpoint_resumed_by_oracle(tmp_path):
+    c = cfg()
+    g = started(c, 5)
+    path = str(tmp_path / "g.ckpt")
+    g.checkpoint_save(path)
+    o = Oracle(c, threads=4)
+    assert o.checkpoint_load(path) == 0
+    compare(o, g, 0.0)
+    g.step(7)
+    o.step(7)
+    compare(g, o, TOL)
+
+
+def test_oracle_checkpoint_resumed_by_library(tmp_path):
+    c = cfg()
+    o = Oracle(c, threads=4)
+    o.load_particles()
+    o.begin()
+    o.step(6)
+    path = str(tmp_path / "o.ckpt")
+    assert o.checkpoint_save(path) == 0
+    g = ps.Simulation(c)
+    g.checkpoint_load(path)
+    assert g.counters()[0] == 6
+    compare(g, o, 0.0)
+    g.step(6)
+    o.step(6)
+    compare(g, o, TOL)
+
+
+def test_fp64_resume_in_fresh_context(tmp_path):
+    c = cfg(preset=1, tile=(4, 4, 8), sort_every=5, inflow_every=3)
+    a = started(c, 4)
+    path = str(tmp_path / "a.ckpt")
+    a.checkpoint_save(path)
+    b = ps.Simulation(c)
+    b.checkpoint_load(path)
+    assert b.counters() == a.counters()
+    a.step(8)
+    b.step(8)
+    compare(a, b, TOL)
+
+
+def test_fp32_deterministic_resume_is_bit_identical(tmp_path):
+    c = cfg(ps.PIC_FP32, preset=1, tile=(4, 4, 8), sort_every=5, inflow_every=3,
+            deposit=ps.PIC_DEPOSIT_DETERMINISTIC)
+    a = started(c, 4)
+    path = str(tmp_path / "a32.ckpt")
+    a.checkpoint_save(path)
+    b = ps.Simulation(c)
+    b.checkpoint_load(path)
+    a.step(8)
+    b.step(8)
+    for n in FIELDS:
+        assert np.array_equal(a.download_field(n), b.download_field(n)), n
+    for sp in (0, 1):
+        ra, rb = a.download_raw32(sp), b.download_raw32(sp)
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x, y), sp
+
+
+def test_fp32_resume_with_other_tiles_continues(tmp_path):
+    c = cfg(ps.PIC_FP32, preset=1, tile=(4, 4, 8), sort_every=5, inflow_every=3,
+            deposit=ps.PIC_DEPOSIT_DETERMINISTIC)
+    a = started(c, 4)
+    path = str(tmp_path / "t.ckpt")
+    a.checkpoint_save(path)
+    d = cfg(ps.PIC_FP32, preset=1, tile=(8, 8, 8), sort_every=5, inflow_every=3,
+            deposit=ps.PIC_DEPOSIT_DETERMINISTIC)
+    b = ps.Simulation(d)
+    b.checkpoint_load(path)
+    a.step(3)
+    b.step(3)
+    # the sort in these steps orders each run by its own tiles: fields and
+    # counts compare directly, particles as sets; deposits reach the planes
+    # through other windows, so fields differ by rounding only
+    for n in FIELDS:
+        e = rel_l2(a.download_field(n), b.download_field(n))
+        assert e <= 1e-6, (n, e)
+    for sp in (0, 1):
+        assert a.particle_count(sp) == b.particle_count(sp)
+        pa, pb = a.download_particles(sp), b.download_particles(sp)
+        pa = pa[np.lexsort(pa[:, ::-1].T)]
+        pb = pb[np.lexsort(pb[:, ::-1].T)]
+        assert rel_l2(pa, pb) <= 1e-6, sp
+
+
+def test_rejected_checkpoints(tmp_path):
+    c = cfg()
+    g = started(c, 2)
+    path = str(tmp_path / "r.ckpt")
+    g.checkpoint_save(path)
+    with pytest.raises(ps.PicError) as e:
+        ps.Simulation(cfg(ps.PIC_FP32)).checkpoint_load(path)
+    assert e.value.code == -7
+    other = cfg()
+    other.seed += 1
+    with pytest.raises(ps.PicError) as e:
+        ps.Simulation(other).checkpoint_load(path)
+    assert e.value.code == -7
+    small = cfg()
+    small.capacity = 16
+    with pytest.raises(ps.PicError) as e:
+        ps.Simulation(small).checkpoint_load(path)
+    assert e.value.code == -4
