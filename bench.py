@@ -19,7 +19,8 @@ correction every step, inflow every 250 steps, tile sort every 20.
 
 --workload checkpoint: pic_checkpoint_save and pic_checkpoint_load of the
 default workload's state after a few steps (FP32 and FP64 forms), in bytes
-per second of checkpoint file, to a file under $TMPDIR (or /tmp).
+per second of checkpoint file, to a file under $TMPDIR (or /tmp), and the
+time of the output() field dump of the FP32 state.
 
 Prints one JSON line: value = particle-steps/s of the chosen form (FP32 by
 default), with the plain FP64 form's rate, the roofline of the dominant
@@ -228,7 +229,8 @@ def run_adjoint(ps, steps=200, ckpt=20, radius=10.0, nquad=1024,
 
 
 def run_checkpoint(ps, steps=3):
-    """Save and load times of the default workload's state in both forms."""
+    """Save and load times of the default workload's state in both forms,
+    and the time of the output() field dump of the FP32 state."""
     import tempfile
     out = {}
     d = os.environ.get("TMPDIR", "/tmp")
@@ -241,6 +243,8 @@ def run_checkpoint(ps, steps=3):
         s.step(steps)
         s.sync()
         fd, path = tempfile.mkstemp(suffix=".ckpt", dir=d)
+        os.close(fd)
+        fd, dump = tempfile.mkstemp(suffix=".dat", dir=d)
         os.close(fd)
         try:
             t0 = time.perf_counter()
@@ -255,9 +259,15 @@ def run_checkpoint(ps, steps=3):
             r.sync()
             t_load = time.perf_counter() - t0
             e1 = r.diagnose()
+            if form == "fp32":
+                t0 = time.perf_counter()
+                r.output_fields(dump)
+                out["output_fields_s"] = time.perf_counter() - t0
+                out["output_fields_bytes"] = os.path.getsize(dump)
             r.close()
         finally:
             os.remove(path)
+            os.remove(dump)
         same = all(e0[k] == e1[k] for k in ("field_energy_e", "field_energy_b",
                                             "charge")) and \
             list(e0["count"]) == list(e1["count"])

@@ -256,6 +256,24 @@ int pic_adj_forward(pic_ctx *ctx, const double params[4], int nsteps,
                     int ckpt_every, double radius, int nquad, double *loss);
 int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]);
 
+/* ---- field output ----
+ *
+ * Cell-centred text dumps of the fields, one line per cell, "%.3E" columns
+ * separated by single spaces, cells in (k,l,m) order with m fastest:
+ * - PIC_OUTPUT_COLUMNS, the reference's output() (:4834-4883): the
+ *   interior cells, 23 columns: x, y, z of the cell corner relative to the
+ *   box centre; ni, ne, ni - ne; E averaged over the 2 nodes along its
+ *   axis; total B averaged over the 4 nodes of its staggered face; the six
+ *   J node values; phi (0: no potential in 3D); |B|; B - B0 averaged;
+ * - PIC_OUTPUT_TECPLOT, output_tecplot() (:4885-4938): a VARIABLES and a
+ *   ZONE header line (step in T and SOLUTIONTIME), cells k, l up to N+2
+ *   inclusive, 24 columns (J averaged along its axis and |J_i - J_e| added),
+ *   an empty line after each (k,l) column.
+ * Either numeric form; the step only enters the Tecplot header. */
+#define PIC_OUTPUT_COLUMNS 0
+#define PIC_OUTPUT_TECPLOT 1
+int pic_output_fields(pic_ctx *ctx, const char *path, int format, int64_t step);
+
 /* ---- checkpoint and resume ----
  *
  * Replaces the reference's suspend_job / continue_job (:7629-7911), which

@@ -78,6 +78,10 @@ int orc_adj_selftest(orc_ctx *o, uint64_t seed, double out[4]);
  * (pic_checkpoint_save / pic_checkpoint_load); pic_oracle_ckpt.c */
 int orc_checkpoint_save(orc_ctx *o, const char *path);
 int orc_checkpoint_load(orc_ctx *o, const char *path);
+
+/* output() / output_tecplot() text dumps (format PIC_OUTPUT_*);
+ * pic_oracle_out.c */
+int orc_output_fields(orc_ctx *o, const char *path, int format, int64_t step);
 int orc_adj_boris_check(const pic_config *cfg, double m, const double f[6],
                         const double v[3], const double lvp[3], double out[2]);
 

@@ -200,6 +200,11 @@ class Simulation:
                                              ctypes.byref(ca), ctypes.byref(cs)))
         return st.value, ca.value, cs.value
 
+    def output_fields(self, path, tecplot=False, step=0):
+        """Cell-centred text dump (output / output_tecplot of the reference)."""
+        self._check(self._L.pic_output_fields(self._ctx, os.fsencode(path),
+                                              1 if tecplot else 0, int(step)))
+
     # -- checkpoint and resume ---------------------------------------------
     def checkpoint_save(self, path):
         """Write the complete state between steps (pic_checkpoint_save)."""

@@ -171,6 +171,7 @@ SIGNATURES = [
     ("pic_adj_forward", ctypes.c_int, [_P, _D, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_int, _D]),
     ("pic_adj_backward", ctypes.c_int, [_P, ctypes.c_double, _D]),
+    ("pic_output_fields", ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int64]),
     ("pic_checkpoint_save", ctypes.c_int, [_P, ctypes.c_char_p]),
     ("pic_checkpoint_load", ctypes.c_int, [_P, ctypes.c_char_p]),
     # include/plasmasim2d.h

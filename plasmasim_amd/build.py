@@ -34,6 +34,7 @@ UNITS = {
     "pic2d_fields.hip": ["-ffp-contract=off"],
     "pic2d_particles.hip": ["-ffp-contract=off"],
     "pic_ckpt.hip": ["-ffp-contract=off"],
+    "pic_output.cpp": ["-ffp-contract=off"],
 }
 
 

@@ -11,9 +11,18 @@
  *
  * Usage: pic_run [--preset N] [--steps K] [--fp64|--fp32] [--det]
  *                [--diag D] [--fused] [--suspend FILE] [--suspend-every S]
- *                [--resume FILE]
+ *                [--resume FILE] [--output-every N] [--tecplot]
+ *                [--thrust-every N] [--thrust-ave] [--version V]
  * --fused calls pic_step() (loops A and B fused into one particle pass)
  * instead of the routine-by-routine loop.
+ *
+ * Output (fdtd :913-927): --output-every N writes the field dump of
+ * output() (:4834-4883; --tecplot: output_tecplot()) to
+ * fdtd<V>-0-0-<c>.dat at steps c with c % N == 0; --thrust-every N appends
+ * thrust_f's line (:5110-5119: c, F[0..26], E[0..5]) to thrust<V>.txt at
+ * those steps; --thrust-ave adds thrust_fave (:5124-5285): F averaged over
+ * every step of each N-step window, written to thrust<V>_ave.txt when
+ * c % N == 0.  <V> is --version (the reference's version number).
  *
  * Suspend and resume (main_func :270-283, fdtd :918-931): with --resume
  * FILE the run continues from FILE when it exists (instead of loading the
@@ -77,6 +86,26 @@ static int step_routines(pic_ctx *ctx, const pic_config *c, int *c_sort,
   return 0;
 }
 
+/* thrust_f's line (:5110-5119): F[15..26] and E[4..5] belong to particles
+ * with flag < 0, which the live code never creates (flag is 1, :510,
+ * :6058), so they are 0 */
+static int thrust_line(const char *path, int64_t c, const double F[27],
+                       const double E[6], int with_e) {
+  FILE *f = fopen(path, "a");
+  if (!f) return PIC_E_IO;
+  int ok = fprintf(f, "%lld", (long long)c) > 0;
+  for (int i = 0; i < 27; ++i) ok = ok && fprintf(f, " %.3E", F[i]) > 0;
+  if (with_e)
+    for (int i = 0; i < 6; ++i) ok = ok && fprintf(f, " %.8E", E[i]) > 0;
+  ok = ok && fprintf(f, "\n") > 0;
+  if (fclose(f) != 0) ok = 0;
+  return ok ? 0 : PIC_E_IO;
+}
+
+static void forces(const pic_diag *d, double F[27]) {
+  for (int i = 0; i < 27; ++i) F[i] = i < 15 ? d->force[i] : 0.0;
+}
+
 /* suspend_job (:7629-7675): the counters of the routine loop go with the
  * state */
 static int suspend(pic_ctx *ctx, const char *path, int fused, int64_t c,
@@ -89,6 +118,7 @@ static int suspend(pic_ctx *ctx, const char *path, int fused, int64_t c,
 int main(int argc, char **argv) {
   int preset = 1, steps = 100, diag = 50, fused = 0;
   int precision = -1, det = 0, suspend_every = 0;
+  int output_every = 0, tecplot = 0, thrust_every = 0, thrust_ave = 0, version = 0;
   const char *suspend_path = NULL, *resume_path = NULL;
   for (int i = 1; i < argc; ++i) {
     if (!strcmp(argv[i], "--preset") && i + 1 < argc) preset = atoi(argv[++i]);
@@ -102,11 +132,19 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[i], "--suspend-every") && i + 1 < argc)
       suspend_every = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--resume") && i + 1 < argc) resume_path = argv[++i];
+    else if (!strcmp(argv[i], "--output-every") && i + 1 < argc)
+      output_every = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--tecplot")) tecplot = 1;
+    else if (!strcmp(argv[i], "--thrust-every") && i + 1 < argc)
+      thrust_every = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--thrust-ave")) thrust_ave = 1;
+    else if (!strcmp(argv[i], "--version") && i + 1 < argc) version = atoi(argv[++i]);
     else {
       fprintf(stderr,
               "usage: %s [--preset N] [--steps K] [--fp64|--fp32] [--det] "
               "[--diag D] [--fused] [--suspend FILE] [--suspend-every S] "
-              "[--resume FILE]\n", argv[0]);
+              "[--resume FILE] [--output-every N] [--tecplot] "
+              "[--thrust-every N] [--thrust-ave] [--version V]\n", argv[0]);
       return 2;
     }
   }
@@ -133,11 +171,36 @@ int main(int argc, char **argv) {
           cfg.nz, cfg.ppc, cfg.precision == PIC_FP64 ? "fp64" : "fp32",
           cfg.deposit == PIC_DEPOSIT_DETERMINISTIC ? "deterministic" : "atomic");
   const double t0 = now();
+  double Fave[27] = {0};
+  char thrust_path[64], ave_path[64];
+  snprintf(thrust_path, sizeof thrust_path, "thrust%d.txt", version);
+  snprintf(ave_path, sizeof ave_path, "thrust%d_ave.txt", version);
   for (int64_t c = c0; c < steps; ++c) {
     if (fused) {
       CHECK(ctx, pic_step(ctx, 1));
     } else {
       CHECK(ctx, step_routines(ctx, &cfg, &c_sort, &c_add));
+    }
+    if (output_every > 0 && c % output_every == 0) {   /* output :913-915 */
+      char name[96];
+      snprintf(name, sizeof name, "fdtd%d-0-0-%lld.dat", version, (long long)c);
+      CHECK(ctx, pic_output_fields(ctx, name, tecplot ? PIC_OUTPUT_TECPLOT
+                                                      : PIC_OUTPUT_COLUMNS, c));
+    }
+    if (thrust_every > 0 && (c % thrust_every == 0 || thrust_ave)) {
+      pic_diag d;
+      double F[27];
+      CHECK(ctx, pic_diagnose(ctx, &d));
+      forces(&d, F);
+      if (c % thrust_every == 0)                        /* thrust_f :917   */
+        CHECK(ctx, thrust_line(thrust_path, c, F, d.energy_box, 1));
+      if (thrust_ave) {                                 /* thrust_fave     */
+        for (int i = 0; i < 27; ++i) Fave[i] += F[i] / thrust_every;
+        if (c % thrust_every == 0) {
+          CHECK(ctx, thrust_line(ave_path, c, Fave, NULL, 0));
+          for (int i = 0; i < 27; ++i) Fave[i] = 0.;
+        }
+      }
     }
     const int last = c + 1 == steps;
     if (suspend_path && ((suspend_every > 0 && (c + 1) % suspend_every == 0) || last))

@@ -58,6 +58,7 @@ _SIG = {
     "orc_adj_force": (ctypes.c_double, [_P, ctypes.c_double, ctypes.c_int]),
     "orc_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "orc_checkpoint_load": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "orc_output_fields": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int64]),
     "orc_adj_selftest": (ctypes.c_int, [_P, ctypes.c_uint64, _D]),
     "orc_adj_boris_check": (ctypes.c_int, [ctypes.POINTER(PicConfig), ctypes.c_double,
                                            _D, _D, _D, _D]),
@@ -214,6 +215,10 @@ class Oracle:
 
     def checkpoint_load(self, path):
         return self._L.orc_checkpoint_load(self._o, os.fsencode(path))
+
+    def output_fields(self, path, tecplot=False, step=0):
+        self._check(self._L.orc_output_fields(self._o, os.fsencode(path),
+                                              1 if tecplot else 0, int(step)))
 
     def adj_force(self, R, nq):
         return self._L.orc_adj_force(self._o, R, nq)
