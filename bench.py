@@ -42,9 +42,11 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 BYTES_PER_PARTICLE = {"fp64": 96, "fp32": 56}
 BYTES_PER_PARTICLE_2D = {"fp64": 80, "fp32": 48}
 # HBM bytes per particle-step of the same pass measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE at this workload, scaled by the counts of a
-# 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md)
+# FETCH_SIZE and WRITE_SIZE at each workload, scaled by the counts of a
+# 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md,
+# profiles/particle_pass_2d3v_2048sq_100ppc.md)
 TRAFFIC_PER_PARTICLE = {"fp64": 101.3, "fp32": 60.3}
+TRAFFIC_PER_PARTICLE_2D = {"fp64": 81.4, "fp32": 48.6}
 
 
 def run_form(ps, form, steps, warmup, workload="3d"):
@@ -86,8 +88,8 @@ def run_form(ps, form, steps, warmup, workload="3d"):
         "rate": rate, "ms_per_step": (t1 - t0) / steps * 1e3,
         "particles": particles, "push_avg_ms": push_avg_s * 1e3,
         "achieved": achieved, "kernel_ms": {k: v[0] / tsteps for k, v in tot.items()},
-        "traffic": (TRAFFIC_PER_PARTICLE[form] * per_launch_particles
-                    if workload == "3d" else None),
+        "traffic": (TRAFFIC_PER_PARTICLE_2D if workload == "2d3v"
+                    else TRAFFIC_PER_PARTICLE)[form] * per_launch_particles,
         "bytes_per_particle": bpp,
     }
 
