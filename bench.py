@@ -49,7 +49,7 @@ TRAFFIC_PER_PARTICLE = {"fp64": 101.3, "fp32": 60.3}
 TRAFFIC_PER_PARTICLE_2D = {"fp64": 81.4, "fp32": 48.6}
 
 
-def run_form(ps, form, steps, warmup, workload="3d"):
+def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0):
     if workload == "2d3v":
         c = ps.preset2d(3)
         c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
@@ -57,6 +57,7 @@ def run_form(ps, form, steps, warmup, workload="3d"):
     else:
         c = ps.preset(2)
         c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
+        c.poisson_every = poisson_every
         sim = ps.Simulation(c)
     sim.load_particles()
     sim.begin()
@@ -289,6 +290,9 @@ def main():
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--workload", choices=["3d", "2d3v", "adjoint", "checkpoint"],
                     default="3d")
+    ap.add_argument("--poisson-every", type=int, default=0, metavar="P",
+                    help="3d: run the 3D Poisson correction every P steps (0: off, "
+                         "as the reference)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -328,10 +332,10 @@ def main():
                                    "x 2 species, file under $TMPDIR"},
             "checkpoint": r}))
         return
-    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload)
+    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload, a.poisson_every)
     other = None
     if a.form == "fp32" and not a.no_fp64:
-        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload)
+        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload, a.poisson_every)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(ps) if a.workload == "3d" else cpu_baseline_2d(ps)
@@ -343,6 +347,8 @@ def main():
         workload = ("3D EM-PIC step, 128^3 cells x 64 ppc x 2 species "
                     "(268,435,456 particles), dipole standoff 32, dt 0.45, "
                     "sort every 20, inflow every 5")
+        if a.poisson_every > 0:
+            workload += f", Poisson correction every {a.poisson_every}"
     out = {
         "metric": "particle-steps/s on 1 MI355X, FP32 vs FP64 form at stated field error",
         "value": main_r["rate"],

@@ -68,6 +68,9 @@ enum pic_field {
   PIC_NI, PIC_NE,                  /* number densities */
   PIC_NFIELDS
 };
+/* download only: the potential of the last Poisson correction in the
+ * current step (zero when poisson_every is 0 or renew_grid ran since) */
+#define PIC_PHI PIC_NFIELDS
 
 /*
  * Runtime replacement of the reference's compile-time configuration
@@ -106,6 +109,14 @@ typedef struct pic_config {
                            reference's key ((k*Ny)+l)*Nz+m (:7488-7497). */
   uint64_t seed;        /* Philox4x32-10 key for loading and inflow */
   int64_t capacity;     /* per-species particle capacity; 0 = automatic */
+  int32_t poisson_every; /* renew_phi3 + sr_phi + E correction (:901-906)
+                            every this many steps, after the E step; 0 =
+                            off, as in the reference (PHI() is empty, :107).
+                            Needs power-of-two nx, ny, nz. */
+  int32_t phi_margin;   /* cells of the Poisson source's margin (reference
+                           Absorb_grid3, :97): the source is zero unless
+                           phi_margin < k-2 < nx-phi_margin (same for l) and
+                           phi_margin <= m-2 < nz-phi_margin (:8340-8344) */
 } pic_config;
 
 /* Diagnostics (reference thrust_f :4940-5122, extended; all FP64). */
@@ -214,6 +225,13 @@ int pic_step(pic_ctx *ctx, int nsteps);
 /* The reference's pre-loop inflow (:673-674) followed by nothing else;
  * call once after pic_load_particles and before the first pic_step. */
 int pic_begin(pic_ctx *ctx);
+/* renew_phi3 (:8315-8665), sr_phi (:1389-1573, with the E correction) and
+ * sr_e: the Dirichlet Poisson correction of E (source q(ni-ne)/eps0 - div E
+ * with the (2,4) divergence, sine transforms along z, y, x, division by the
+ * 7-point Laplacian symbol, inverse transforms; E += (2,4) gradient of phi).
+ * PIC_E_STATE when poisson_every is 0 (no buffers). */
+int pic_poisson(pic_ctx *ctx);
+
 /* Step counter (c) and cadence counters (c_add, c_thrust).  pic_step keeps
  * them; a host loop that calls the routines itself keeps its own (fdtd's
  * locals, :630-686) and sets them before pic_checkpoint_save. */
@@ -264,7 +282,8 @@ int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]);
  *   interior cells, 23 columns: x, y, z of the cell corner relative to the
  *   box centre; ni, ne, ni - ne; E averaged over the 2 nodes along its
  *   axis; total B averaged over the 4 nodes of its staggered face; the six
- *   J node values; phi (0: no potential in 3D); |B|; B - B0 averaged;
+ *   J node values; phi (PIC_PHI: 0 unless the Poisson correction ran in
+ *   this step); |B|; B - B0 averaged;
  * - PIC_OUTPUT_TECPLOT, output_tecplot() (:4885-4938): a VARIABLES and a
  *   ZONE header line (step in T and SOLUTIONTIME), cells k, l up to N+2
  *   inclusive, 24 columns (J averaged along its axis and |J_i - J_e| added),

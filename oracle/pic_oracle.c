@@ -284,9 +284,10 @@ int orc_upload_field(orc_ctx *o, int which, const double *host) {
 }
 
 int orc_download_field(orc_ctx *o, int which, double *host) {
-  if (which < 0 || which >= PIC_NFIELDS) return PIC_E_ARG;
+  if (which < 0 || which > PIC_PHI) return PIC_E_ARG;
   size_t n = (size_t)o->SX * o->SY * o->SZ;
-  for (size_t i = 0; i < n; i++) host[i] = *field_ptr(&o->grid[i], which);
+  for (size_t i = 0; i < n; i++)
+    host[i] = which == PIC_PHI ? o->grid[i].phi : *field_ptr(&o->grid[i], which);
   return 0;
 }
 
@@ -1128,6 +1129,8 @@ int orc_step(orc_ctx *o, int nsteps) {
     orc_fold_current(o);
     orc_renew_e(o);
     orc_halo_e(o);
+    if (c->poisson_every > 0 && o->step % c->poisson_every == 0) /* PHI() :901 */
+      if ((rc = orc_poisson(o))) return rc;
     orc_particle_boundary(o);
     o->c_sort++;
     o->c_add++;

@@ -82,6 +82,9 @@ int orc_checkpoint_load(orc_ctx *o, const char *path);
 /* output() / output_tecplot() text dumps (format PIC_OUTPUT_*);
  * pic_oracle_out.c */
 int orc_output_fields(orc_ctx *o, const char *path, int format, int64_t step);
+
+/* renew_phi3 + sr_phi + E correction + sr_e; pic_oracle_phi3.c */
+int orc_poisson(orc_ctx *o);
 int orc_adj_boris_check(const pic_config *cfg, double m, const double f[6],
                         const double v[3], const double lvp[3], double out[2]);
 

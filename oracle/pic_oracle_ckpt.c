@@ -95,7 +95,8 @@ static int same_physics(const pic_config *a, const pic_config *b) {
            a->dipole_rmax == b->dipole_rmax && a->sigma_amp == b->sigma_amp &&
            a->inflow_every == b->inflow_every &&
            a->sort_every == b->sort_every && a->refill_far == b->refill_far &&
-           a->precision == b->precision && a->seed == b->seed;
+           a->precision == b->precision && a->seed == b->seed &&
+           a->poisson_every == b->poisson_every && a->phi_margin == b->phi_margin;
   for (int s = 0; s < 2; s++)
     ok = ok && a->mass[s] == b->mass[s] && a->weight[s] == b->weight[s] &&
          a->vth[s] == b->vth[s] && a->periodic[s] == b->periodic[s];

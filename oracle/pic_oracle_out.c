@@ -3,7 +3,7 @@
  *
  * Restates output() (PIC/3d_pic458_domain.c :4834-4883) and
  * output_tecplot() (:4885-4938) for one rank (myrank 0, one process per
- * axis) on the oracle's Grid records; phi is printed as stored (0).
+ * axis) on the oracle's Grid records; phi is printed as stored.
  */
 #include <math.h>
 #include <stdio.h>

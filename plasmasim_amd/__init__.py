@@ -161,6 +161,9 @@ class Simulation:
     def sort(self, sp):
         self._check(self._L.pic_sort(self._ctx, sp))
 
+    def poisson(self):
+        self._check(self._L.pic_poisson(self._ctx))
+
     def begin(self):
         self._check(self._L.pic_begin(self._ctx))
 

@@ -21,6 +21,7 @@ PIC_ION, PIC_ELECTRON = 0, 1
 FIELDS = ["ex", "ey", "ez", "bx", "by", "bz", "b0x", "b0y", "b0z",
           "jix", "jiy", "jiz", "jex", "jey", "jez", "ni", "ne"]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+FIELD_ID["phi"] = len(FIELDS)  # PIC_PHI: download only
 TIMING = {"push": 0, "field_b": 1, "field_e": 2, "sort": 3, "inflow": 4,
           "other": 5}
 
@@ -45,6 +46,7 @@ class PicConfig(ctypes.Structure):
         ("precision", ctypes.c_int32), ("deposit", ctypes.c_int32),
         ("tile", ctypes.c_int32 * 3),
         ("seed", ctypes.c_uint64), ("capacity", ctypes.c_int64),
+        ("poisson_every", ctypes.c_int32), ("phi_margin", ctypes.c_int32),
     ]
 
     def copy(self):
@@ -172,6 +174,7 @@ SIGNATURES = [
                                        ctypes.c_double, ctypes.c_int, _D]),
     ("pic_adj_backward", ctypes.c_int, [_P, ctypes.c_double, _D]),
     ("pic_output_fields", ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int64]),
+    ("pic_poisson", ctypes.c_int, [_P]),
     ("pic_checkpoint_save", ctypes.c_int, [_P, ctypes.c_char_p]),
     ("pic_checkpoint_load", ctypes.c_int, [_P, ctypes.c_char_p]),
     # include/plasmasim2d.h

@@ -35,6 +35,7 @@ UNITS = {
     "pic2d_particles.hip": ["-ffp-contract=off"],
     "pic_ckpt.hip": ["-ffp-contract=off"],
     "pic_output.cpp": ["-ffp-contract=off"],
+    "pic_poisson.hip": ["-ffp-contract=off"],
 }
 
 

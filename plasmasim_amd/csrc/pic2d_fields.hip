@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include "pic2d_internal.h"
+#include "pic_dst.h"
 
 namespace pic2d {
 
@@ -287,90 +288,6 @@ __global__ void k_phi_rhs(Geom2 g, const T *__restrict__ ni,
 }
 
 template <typename T>
-struct Cplx {
-  T re, im;
-};
-
-// DST-I of two lines per workgroup through one complex FFT of length 2n:
-// z = a + i b, odd-extended (z_0 = z_n = 0, z_{2n-j} = -z_j), so that
-// FFT(z)_k = -2i S_a(k) + 2 S_b(k) with S(k) = sum_{j=1}^{n-1} y_j
-// sin(pi j k / n).  Element j of line l is data[l * sline + j * selem];
-// y_0 is the Dirichlet point and is taken as zero.  Radix-2
-// decimation-in-time on LDS, input in bit-reversed order.
-//   mode 0: out = S                       (forward)
-//   mode 1: out = S / A(l, k), 0 if A = 0  (forward, divided by the symbol
-//           A = -(sym_l[l] + sym_k[k]) / dq of the 5-point Laplacian)
-//   mode 2: out = 2 S / n                 (inverse)
-template <typename T>
-__global__ __launch_bounds__(256) void k_dst2(T *__restrict__ data, int n,
-                                              int logL, int nlines,
-                                              int64_t sline, int64_t selem,
-                                              const Cplx<T> *__restrict__ tw,
-                                              int mode,
-                                              const double *__restrict__ sym_l,
-                                              const double *__restrict__ sym_k,
-                                              double dq) {
-  extern __shared__ unsigned char lds_raw[];
-  T *re = (T *)lds_raw;
-  const int L = 2 * n;
-  T *im = re + L;
-  const int l0 = 2 * blockIdx.x;
-  const bool has_b = l0 + 1 < nlines;
-  const T *A = data + (int64_t)l0 * sline;
-  const T *B = data + (int64_t)(l0 + 1) * sline;
-  for (int j = threadIdx.x; j < L; j += blockDim.x) {
-    T a = 0, b = 0;
-    if (j != 0 && j != n) {
-      const int s = j < n ? j : L - j;
-      a = A[(int64_t)s * selem];
-      b = has_b ? B[(int64_t)s * selem] : (T)0;
-      if (j > n) {
-        a = -a;
-        b = -b;
-      }
-    }
-    const int r = (int)(__brev((unsigned)j) >> (32 - logL));
-    re[r] = a;
-    im[r] = b;
-  }
-  __syncthreads();
-  for (int h = 1; h < L; h <<= 1) {
-    const int tstride = L / (2 * h);
-    for (int t = threadIdx.x; t < L / 2; t += blockDim.x) {
-      const int j = t & (h - 1);
-      const int i0 = ((t - j) << 1) + j, i1 = i0 + h;
-      const Cplx<T> w = tw[j * tstride];
-      const T vr = re[i1] * w.re - im[i1] * w.im;
-      const T vi = re[i1] * w.im + im[i1] * w.re;
-      const T ur = re[i0], ui = im[i0];
-      re[i0] = ur + vr;
-      im[i0] = ui + vi;
-      re[i1] = ur - vr;
-      im[i1] = ui - vi;
-    }
-    __syncthreads();
-  }
-  T *OA = data + (int64_t)l0 * sline;
-  T *OB = data + (int64_t)(l0 + 1) * sline;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    T sa = -im[k] / (T)2, sb = re[k] / (T)2;
-    if (mode == 1) {
-      const double Aa = -(sym_l[l0] + sym_k[k]) / dq;
-      sa = Aa == 0. ? (T)0 : (T)(sa / (T)Aa);
-      if (has_b) {
-        const double Ab = -(sym_l[l0 + 1] + sym_k[k]) / dq;
-        sb = Ab == 0. ? (T)0 : (T)(sb / (T)Ab);
-      }
-    } else if (mode == 2) {
-      sa = (T)2 * sa / (T)n;
-      sb = (T)2 * sb / (T)n;
-    }
-    OA[(int64_t)k * selem] = sa;
-    if (has_b) OB[(int64_t)k * selem] = sb;
-  }
-}
-
-template <typename T>
 __global__ void k_phi_apply(Geom2 g, const T *__restrict__ rho,
                             T *__restrict__ ex, T *__restrict__ ez,
                             T *__restrict__ phi, T dx) {
@@ -387,41 +304,20 @@ __global__ void k_phi_apply(Geom2 g, const T *__restrict__ rho,
   ez[i] += (ph - pzm) / dx;
 }
 
-static int ilog2(int n) {
-  int l = 0;
-  while ((1 << l) < n) ++l;
-  return l;
-}
-
 // twiddles exp(-2 pi i t / L), t < L/2, and the symbol tables
 // sin^2(pi k / (2 N)) of both axes, computed on the host in double as the
 // reference computes its symbol (:2679-2686)
 int init_poisson(pic2d_ctx *ctx) {
   if (ctx->cfg.poisson_every <= 0) return 0;
   const int NX = ctx->g.NX, NZ = ctx->g.NZ;
-  const double Pi = 3.14159265358979323846;
   const size_t esz = ctx->f64 ? sizeof(double) : sizeof(float);
   P2_HIP(ctx, hipMalloc(&ctx->rho, (size_t)NX * NZ * esz));
   for (int axis = 0; axis < 2; ++axis) {
     const int N = axis == 0 ? NX : NZ;
-    const int L = 2 * N;
-    // layout: L/2 complex twiddles, then N doubles of the symbol
-    const size_t tw_bytes = (size_t)(L / 2) * 2 * esz;
-    const size_t bytes = tw_bytes + (size_t)N * sizeof(double);
-    unsigned char *h = (unsigned char *)malloc(bytes);
+    size_t bytes = 0, tw_bytes = 0;
+    unsigned char *h = ctx->f64 ? pic::dst_axis_tables<double>(N, &bytes, &tw_bytes)
+                                : pic::dst_axis_tables<float>(N, &bytes, &tw_bytes);
     if (!h) return err(ctx, PIC_E_NOMEM, "poisson tables");
-    for (int t = 0; t < L / 2; ++t) {
-      const double a = 2.0 * Pi * (double)t / (double)L;
-      if (ctx->f64) {
-        ((double *)h)[2 * t] = cos(a);
-        ((double *)h)[2 * t + 1] = -sin(a);
-      } else {
-        ((float *)h)[2 * t] = (float)cos(a);
-        ((float *)h)[2 * t + 1] = (float)-sin(a);
-      }
-    }
-    double *sym = (double *)(h + tw_bytes);
-    for (int k = 0; k < N; ++k) sym[k] = pow(sin(Pi * (double)k / (double)N / 2.), 2);
     void **dst = axis == 0 ? &ctx->tw_x : &ctx->tw_z;
     hipError_t e = hipMalloc(dst, bytes);
     if (e == hipSuccess) e = hipMemcpy(*dst, h, bytes, hipMemcpyHostToDevice);
@@ -444,6 +340,8 @@ static int poisson_t(pic2d_ctx *ctx) {
       plane<T>(ctx, PIC_EZ), rho, c.phi_lo[0], c.phi_hi[0], c.phi_lo[1],
       c.phi_hi[1], (T)c.q, (T)c.eps0, (T)c.dx);
   P2_LAUNCH_CHECK(ctx);
+  using pic::Cplx;
+  using pic::k_dst;
   const Cplx<T> *twx = (const Cplx<T> *)ctx->tw_x;
   const Cplx<T> *twz = (const Cplx<T> *)ctx->tw_z;
   const double *symx = (const double *)((const char *)ctx->tw_x + (size_t)NX * sizeof(Cplx<T>));
@@ -451,16 +349,17 @@ static int poisson_t(pic2d_ctx *ctx) {
   const double dq = pow(c.dx / 2., 2);
   const size_t ldsz = (size_t)4 * NZ * sizeof(T), ldsx = (size_t)4 * NX * sizeof(T);
   const unsigned bz = (unsigned)div_up(NX, 2), bx = (unsigned)div_up(NZ, 2);
+  const int lz = pic::ilog2_ceil(2 * NZ), lx = pic::ilog2_ceil(2 * NX);
   // forward along z (lines k, contiguous), then along x (lines m, stride
   // NZ) with the division, inverse along x, inverse along z
-  k_dst2<T><<<bz, 256, ldsz, st>>>(rho, NZ, ilog2(2 * NZ), NX, NZ, 1, twz, 0,
-                                   nullptr, nullptr, 0.);
-  k_dst2<T><<<bx, 256, ldsx, st>>>(rho, NX, ilog2(2 * NX), NZ, 1, NZ, twx, 1,
-                                   symz, symx, dq);
-  k_dst2<T><<<bx, 256, ldsx, st>>>(rho, NX, ilog2(2 * NX), NZ, 1, NZ, twx, 2,
-                                   nullptr, nullptr, 0.);
-  k_dst2<T><<<bz, 256, ldsz, st>>>(rho, NZ, ilog2(2 * NZ), NX, NZ, 1, twz, 2,
-                                   nullptr, nullptr, 0.);
+  k_dst<T><<<bz, 256, ldsz, st>>>(rho, NZ, lz, NX, NX, NZ, 0, 1, twz, 0, nullptr,
+                                  nullptr, 0.);
+  k_dst<T><<<bx, 256, ldsx, st>>>(rho, NX, lx, NZ, NZ, 1, 0, NZ, twx, 1, symz, symx,
+                                  dq);
+  k_dst<T><<<bx, 256, ldsx, st>>>(rho, NX, lx, NZ, NZ, 1, 0, NZ, twx, 2, nullptr,
+                                  nullptr, 0.);
+  k_dst<T><<<bz, 256, ldsz, st>>>(rho, NZ, lz, NX, NX, NZ, 0, 1, twz, 2, nullptr,
+                                  nullptr, 0.);
   P2_LAUNCH_CHECK(ctx);
   k_phi_apply<T><<<grid, kBlock, 0, st>>>(g, rho, plane<T>(ctx, PIC_EX),
                                           plane<T>(ctx, PIC_EZ),

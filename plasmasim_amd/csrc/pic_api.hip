@@ -123,6 +123,8 @@ static void fill_common(pic_config *c, int n, int ppc, double mi, double V,
   c->deposit = PIC_DEPOSIT_ATOMIC;
   c->tile[0] = c->tile[1] = 4;
   c->tile[2] = 8;
+  c->poisson_every = 0;  // PHI() is compiled out in the reference (:107)
+  c->phi_margin = bw;    // Absorb_grid3 = Absorb_grid2, the ballistic width
 }
 
 extern "C" size_t pic_sizeof_config(void) { return sizeof(pic_config); }
@@ -198,6 +200,19 @@ static int validate(const pic_config *c, char *err, size_t n) {
     snprintf(err, n, "cadences must be >= 1");
     return PIC_E_ARG;
   }
+  if (c->poisson_every < 0 || c->phi_margin < 0) {
+    snprintf(err, n, "poisson_every and phi_margin must be >= 0");
+    return PIC_E_ARG;
+  }
+  if (c->poisson_every > 0)
+    for (int a = 0; a < 3; ++a) {
+      const int N = a == 0 ? c->nx : a == 1 ? c->ny : c->nz;
+      if (N < 2 || (N & (N - 1)) != 0) {
+        snprintf(err, n, "the Poisson correction needs power-of-two sizes (axis %d: %d)",
+                 a, N);
+        return PIC_E_ARG;
+      }
+    }
   for (int a = 0; a < 3; ++a) {
     const int N = a == 0 ? c->nx : a == 1 ? c->ny : c->nz;
     if (c->tile[a] < 0 || (c->tile[a] > 0 && N % c->tile[a] != 0)) {
@@ -282,6 +297,7 @@ static int host_fields_b0(const pic_ctx *ctx, double *b0) {
 }
 
 static void free_all(pic_ctx *ctx) {
+  poisson_free(ctx);
   if (ctx->field_mem) (void)hipFree(ctx->field_mem);
   if (ctx->sig_mem) (void)hipFree(ctx->sig_mem);
   for (int s = 0; s < 2; ++s)
@@ -445,6 +461,15 @@ extern "C" int pic_create(const pic_config *cfg, pic_ctx **out) {
     for (int j = 0; j < 2; ++j) CREATE_HIP(hipEventCreate(&ctx->tev[j][i]));
   CREATE_HIP(hipStreamSynchronize(ctx->stream));
 #undef CREATE_HIP
+  if (cfg->poisson_every > 0) {
+    int rc2 = poisson_init(ctx);
+    if (rc2) {
+      fprintf(stderr, "pic_create: %s\n", ctx->err);
+      free_all(ctx);
+      delete ctx;
+      return rc2;
+    }
+  }
   if (!f64) {
     int rc2 = f32::init_state(ctx);
     if (rc2) {
@@ -523,9 +548,10 @@ extern "C" int pic_upload_field(pic_ctx *ctx, int which, const double *host) {
 }
 
 extern "C" int pic_download_field(pic_ctx *ctx, int which, double *host) {
-  if (!ctx || !host || which < 0 || which >= PIC_NFIELDS) return PIC_E_ARG;
+  if (!ctx || !host || which < 0 || which > PIC_PHI) return PIC_E_ARG;
   const Geom &g = ctx->geom;
   const int64_t nc = (int64_t)g.SX * g.SY * g.SZ;
+  if (which == PIC_PHI) return poisson_download_phi(ctx, host);
   if (!is_f64(ctx)) {
     int rc = f32::finish_deposit(ctx);
     if (rc) return rc;
@@ -638,8 +664,20 @@ extern "C" int pic_renew_grid(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   timing_begin(ctx);
   int rc = DISPATCH(renew_grid, ctx);
+  if (!rc && ctx->phi) rc = poisson_zero_phi(ctx);  // renew_grid zeroes phi (:2552)
   timing_end(ctx, PIC_T_OTHER);
   return rc;
+}
+
+extern "C" int pic_poisson(pic_ctx *ctx) {
+  if (!ctx) return PIC_E_ARG;
+  if (!ctx->phi)
+    return set_error(ctx, PIC_E_STATE, "poisson: disabled (poisson_every = 0)");
+  timing_begin(ctx);
+  int rc = poisson(ctx);
+  timing_end(ctx, PIC_T_FIELD_E);
+  if (rc) return rc;
+  return pic_halo_e(ctx);
 }
 
 extern "C" int pic_renew_b_half(pic_ctx *ctx) {
@@ -802,6 +840,8 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
     TRY(pic_fold_current(ctx));
     TRY(pic_renew_e(ctx));
     TRY(pic_halo_e(ctx));
+    if (c.poisson_every > 0 && ctx->step % c.poisson_every == 0)
+      TRY(pic_poisson(ctx));  // PHI(renew_phi3; sr_phi; sr_e) :901-906
     ctx->c_sort++;
     ctx->c_add++;
     ctx->step++;

@@ -107,6 +107,8 @@ pic_config canon(const pic_config &c) {
   z.refill_far = c.refill_far;
   z.precision = c.precision;
   z.seed = c.seed;
+  z.poisson_every = c.poisson_every;
+  z.phi_margin = c.phi_margin;
   return z;
 }
 

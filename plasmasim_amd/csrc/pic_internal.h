@@ -117,6 +117,14 @@ struct pic_ctx {
   void *adj = nullptr;
   int adj_nosort = 0;  // the adjoint's forward runs without the sort
 
+  // 3D Poisson correction (poisson_every > 0): phi (a field plane of the
+  // form's type, guard-inclusive), the transform buffer rho (nx ny nz) and
+  // the transform tables of the three axes plus the per-line symbol of the
+  // x pass (pic_poisson.hip)
+  void *phi = nullptr, *rho = nullptr;
+  void *dst_tab[3] = {nullptr, nullptr, nullptr};
+  double *sym_yz = nullptr;
+
   // step counters (reference fdtd(): c, c_add, c_thrust)
   int64_t step = 0;
   int32_t c_add = 0, c_sort = 0;
@@ -250,6 +258,13 @@ int select_flagged(pic_ctx *ctx, const uint8_t *flags, uint32_t *idx_out,
 int sort_keys(pic_ctx *ctx, uint32_t *keys_in, uint32_t *keys_out,
               uint32_t *idx_out, int64_t n, int end_bit);
 int ensure_prim_tmp(pic_ctx *ctx, size_t bytes);
+// 3D Poisson correction (pic_poisson.hip)
+int poisson_init(pic_ctx *ctx);
+void poisson_free(pic_ctx *ctx);
+int poisson_zero_phi(pic_ctx *ctx);
+int poisson(pic_ctx *ctx);  // renew_phi3 + sr_phi + E correction, no sr_e
+int poisson_download_phi(pic_ctx *ctx, double *host);
+
 // tile_start[0..nt] from keys sorted ascending (cpt cells per tile; keys
 // >= nt * cpt are dropped slots and land in tile nt)
 int tile_bounds(pic_ctx *ctx, const uint32_t *skeys, int64_t n, uint32_t cpt,

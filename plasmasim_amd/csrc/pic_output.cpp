@@ -4,8 +4,9 @@
 // The planes are downloaded as doubles (either numeric form), the columns
 // are evaluated with the reference's expressions in its order of
 // operations, and the text of the k-planes is formatted by several host
-// threads and written in order.  phi is 0: the 3D step keeps no potential
-// (the reference's renew_phi3 is compiled out).
+// threads and written in order.  phi is the potential of the last Poisson
+// correction (0 unless poisson_every enables renew_phi3, which the
+// reference compiles out).
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -21,7 +22,7 @@ namespace {
 
 struct Planes {
   int SX, SY, SZ;
-  std::vector<double> f[PIC_NFIELDS];
+  std::vector<double> f[PIC_NFIELDS + 1];  // + PIC_PHI
   double at(int w, int k, int l, int m) const {
     return f[w][((size_t)k * SY + l) * SZ + m];
   }
@@ -64,7 +65,7 @@ void line_columns(const Planes &p, const pic_config &c, int k, int l, int m,
   v[10] = by;
   v[11] = bz;
   for (int j = 0; j < 6; ++j) v[12 + j] = p.at(PIC_JIX + j, k, l, m);
-  v[18] = 0.;  // phi
+  v[18] = p.at(PIC_PHI, k, l, m);
   v[19] = sqrt(bx * bx + by * by + bz * bz);
   v[20] = bx - b0x;
   v[21] = by - b0y;
@@ -117,7 +118,7 @@ void line_tecplot(const Planes &p, const pic_config &c, int k, int l, int m,
   }
   v[18] = sqrt((ji[0] - je[0]) * (ji[0] - je[0]) + (ji[1] - je[1]) * (ji[1] - je[1]) +
                (ji[2] - je[2]) * (ji[2] - je[2]));
-  v[19] = 0.;  // phi
+  v[19] = p.at(PIC_PHI, k, l, m);
   v[20] = sqrt(bx * bx + by * by + bz * bz);
   v[21] = bx - b0x;
   v[22] = by - b0y;
@@ -141,11 +142,11 @@ extern "C" int pic_output_fields(pic_ctx *ctx, const char *path, int format,
   p.SZ = c.nz + 4;
   const size_t nc = (size_t)p.SX * p.SY * p.SZ;
   try {
-    for (int w = 0; w < PIC_NFIELDS; ++w) p.f[w].resize(nc);
+    for (int w = 0; w <= PIC_PHI; ++w) p.f[w].resize(nc);
   } catch (...) {
     return PIC_E_NOMEM;
   }
-  for (int w = 0; w < PIC_NFIELDS; ++w)
+  for (int w = 0; w <= PIC_PHI; ++w)
     if ((rc = pic_download_field(ctx, w, p.f[w].data()))) return rc;
 
   const bool tec = format == PIC_OUTPUT_TECPLOT;

@@ -13,8 +13,11 @@
  *                [--diag D] [--fused] [--suspend FILE] [--suspend-every S]
  *                [--resume FILE] [--output-every N] [--tecplot]
  *                [--thrust-every N] [--thrust-ave] [--version V]
+ *                [--poisson-every P]
  * --fused calls pic_step() (loops A and B fused into one particle pass)
- * instead of the routine-by-routine loop.
+ * instead of the routine-by-routine loop.  --poisson-every P enables the 3D
+ * Poisson correction (renew_phi3, compiled out in the reference) every P
+ * steps.
  *
  * Output (fdtd :913-927): --output-every N writes the field dump of
  * output() (:4834-4883; --tecplot: output_tecplot()) to
@@ -56,8 +59,8 @@ static double now(void) {
 }
 
 /* one reference step: fdtd() body :687-949 */
-static int step_routines(pic_ctx *ctx, const pic_config *c, int *c_sort,
-                         int *c_add) {
+static int step_routines(pic_ctx *ctx, const pic_config *c, int64_t step,
+                         int *c_sort, int *c_add) {
   int rc;
   if ((rc = pic_renew_grid(ctx))) return rc;                    /* :688 */
   if (*c_sort > c->sort_every - 1) {                            /* :694 */
@@ -80,6 +83,8 @@ static int step_routines(pic_ctx *ctx, const pic_config *c, int *c_sort,
   if ((rc = pic_fold_current(ctx))) return rc;                  /* :893 */
   if ((rc = pic_renew_e(ctx))) return rc;                       /* :895 */
   if ((rc = pic_halo_e(ctx))) return rc;                        /* :899 */
+  if (c->poisson_every > 0 && step % c->poisson_every == 0)      /* PHI(), :901 */
+    if ((rc = pic_poisson(ctx))) return rc;  /* renew_phi3, sr_phi, sr_e */
   if ((rc = pic_particle_boundary(ctx))) return rc;             /* :909-911 */
   (*c_sort)++;
   (*c_add)++;
@@ -119,6 +124,7 @@ int main(int argc, char **argv) {
   int preset = 1, steps = 100, diag = 50, fused = 0;
   int precision = -1, det = 0, suspend_every = 0;
   int output_every = 0, tecplot = 0, thrust_every = 0, thrust_ave = 0, version = 0;
+  int poisson_every = -1;
   const char *suspend_path = NULL, *resume_path = NULL;
   for (int i = 1; i < argc; ++i) {
     if (!strcmp(argv[i], "--preset") && i + 1 < argc) preset = atoi(argv[++i]);
@@ -139,12 +145,15 @@ int main(int argc, char **argv) {
       thrust_every = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--thrust-ave")) thrust_ave = 1;
     else if (!strcmp(argv[i], "--version") && i + 1 < argc) version = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--poisson-every") && i + 1 < argc)
+      poisson_every = atoi(argv[++i]);
     else {
       fprintf(stderr,
               "usage: %s [--preset N] [--steps K] [--fp64|--fp32] [--det] "
               "[--diag D] [--fused] [--suspend FILE] [--suspend-every S] "
               "[--resume FILE] [--output-every N] [--tecplot] "
-              "[--thrust-every N] [--thrust-ave] [--version V]\n", argv[0]);
+              "[--thrust-every N] [--thrust-ave] [--version V] "
+              "[--poisson-every P]\n", argv[0]);
       return 2;
     }
   }
@@ -153,6 +162,7 @@ int main(int argc, char **argv) {
   CHECK(ctx, pic_config_preset(preset, &cfg));
   if (precision >= 0) cfg.precision = precision;
   if (det) cfg.deposit = PIC_DEPOSIT_DETERMINISTIC;
+  if (poisson_every >= 0) cfg.poisson_every = poisson_every;
   CHECK(ctx, pic_create(&cfg, &ctx));                 /* init_grid :291 */
   int c_sort = 0, c_add = 0;
   int64_t c0 = 0;
@@ -179,7 +189,7 @@ int main(int argc, char **argv) {
     if (fused) {
       CHECK(ctx, pic_step(ctx, 1));
     } else {
-      CHECK(ctx, step_routines(ctx, &cfg, &c_sort, &c_add));
+      CHECK(ctx, step_routines(ctx, &cfg, c, &c_sort, &c_add));
     }
     if (output_every > 0 && c % output_every == 0) {   /* output :913-915 */
       char name[96];

@@ -59,6 +59,7 @@ _SIG = {
     "orc_checkpoint_save": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "orc_checkpoint_load": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "orc_output_fields": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int64]),
+    "orc_poisson": (ctypes.c_int, [_P]),
     "orc_adj_selftest": (ctypes.c_int, [_P, ctypes.c_uint64, _D]),
     "orc_adj_boris_check": (ctypes.c_int, [ctypes.POINTER(PicConfig), ctypes.c_double,
                                            _D, _D, _D, _D]),
@@ -215,6 +216,9 @@ class Oracle:
 
     def checkpoint_load(self, path):
         return self._L.orc_checkpoint_load(self._o, os.fsencode(path))
+
+    def poisson(self):
+        self._check(self._L.orc_poisson(self._o))
 
     def output_fields(self, path, tecplot=False, step=0):
         self._check(self._L.orc_output_fields(self._o, os.fsencode(path),

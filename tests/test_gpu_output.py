@@ -35,9 +35,12 @@ def test_identical_state_gives_identical_file(tmp_path, tec):
     assert pg.read_bytes() == po.read_bytes()
 
 
-def test_after_steps_matches_oracle_to_printed_precision(tmp_path):
+@pytest.mark.parametrize("poisson", [0, 1])
+def test_after_steps_matches_oracle_to_printed_precision(tmp_path, poisson):
     c = ps.preset(0)
     c.inflow_every = 3
+    c.poisson_every = poisson  # 1: the phi column carries the potential
+    c.phi_margin = 0
     g, o = ps.Simulation(c), Oracle(c, threads=4)
     for s in (g, o):
         s.load_particles()
@@ -52,6 +55,7 @@ def test_after_steps_matches_oracle_to_printed_precision(tmp_path):
     # rounding boundary
     assert np.allclose(a, b, rtol=2e-3, atol=0)
     assert np.mean(a == b) > 0.999
+    assert (np.abs(a[:, 18]).max() > 0) == bool(poisson)
 
 
 def test_fp32_form_dump(tmp_path):
