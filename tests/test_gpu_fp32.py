@@ -200,3 +200,33 @@ def test_multi_step_runs_with_sort_and_inflow():
         n1 = s.particle_count(sp)
         assert abs(n1 - n0[sp]) < 0.05 * n0[sp]
     assert np.isfinite(d["field_energy_e"]) and d["field_energy_e"] > 0
+
+
+def test_config5_largest_workload_runs():
+    """256^3 x 32 ppc x 2 species (1,073,741,824 particles, m_i/m_e = 400,
+    drift 0.05, dipole tilted 30 degrees) in the FP32 form: load, pre-loop
+    inflow and 21 steps (an inflow every 5, the sort at step 20) keep the
+    particle counts within the inflow/outflow balance and every diagnostic
+    finite."""
+    import time
+    c = ps.preset(5)
+    s = ps.Simulation(c)
+    s.load_particles()
+    s.begin()
+    n0 = [s.particle_count(sp) for sp in (0, 1)]
+    assert n0[0] == n0[1] >= 256**3 * 32
+    s.step(1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(20)
+    s.sync()
+    dt = (time.perf_counter() - t0) / 20
+    d = s.diagnose()
+    print(f"config 5: {sum(n0)} particles, {dt * 1e3:.1f} ms per step, "
+          f"{sum(n0) / dt:.3e} particle-steps/s")
+    for k in ("field_energy_e", "field_energy_b1", "charge", "gauss_l2"):
+        assert np.isfinite(d[k]), k
+    assert np.all(np.isfinite(d["kinetic"]))
+    for sp in (0, 1):
+        assert abs(d["count"][sp] - n0[sp]) < 0.02 * n0[sp]
+    s.close()
