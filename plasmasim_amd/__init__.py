@@ -203,6 +203,11 @@ class Simulation:
                                              ctypes.byref(ca), ctypes.byref(cs)))
         return st.value, ca.value, cs.value
 
+    def set_counters(self, step, c_add, c_sort):
+        """Set the step and cadence counters (as a restored run would)."""
+        self._check(self._L.pic_set_counters(self._ctx, int(step), int(c_add),
+                                             int(c_sort)))
+
     def output_fields(self, path, tecplot=False, step=0):
         """Cell-centred text dump (output / output_tecplot of the reference)."""
         self._check(self._L.pic_output_fields(self._ctx, os.fsencode(path),

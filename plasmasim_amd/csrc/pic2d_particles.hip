@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void k_pass64(Geom2 g, Tile2 tg, F64Fields F,
   __shared__ Win2 sw;
   __shared__ int64_t s_range[2];
   const int tid = threadIdx.x;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int bid = xcd_remap(blockIdx.x, tr.ntiles, (int)gridDim.x);
   const double dx = kp.dx;
   if (tid == 0) {
     int64_t s, e;
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
   __shared__ Win2 sw;
   __shared__ int64_t s_range[2];
   const int tid = threadIdx.x;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int bid = xcd_remap(blockIdx.x, tr.ntiles, (int)gridDim.x);
   if (tid == 0) {
     int64_t s, e;
     Win2 w;

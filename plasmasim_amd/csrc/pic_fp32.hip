@@ -347,46 +347,100 @@ __device__ inline bool in_win_s(const WinS &w, int k, int l, int m) {
          (unsigned)(m - w.vm0) <= (unsigned)(w.vm1 - w.vm0);
 }
 
-// J (PUSH_J) or n (DRIFT_N) deposit of one particle at cell (k,l,m),
-// offsets (ox,oy,oz)
-template <bool DET, typename A>
-__device__ __forceinline__ void deposit(const Geom &g, const WinS &w, const Walk &wk,
-                                        char *acc, bool lds_ok, bool current,
-                                        int k, int l, int m, float ox, float oy,
-                                        float oz, float vx, float vy, float vz,
-                                        const Push32 &kp, A *jx, A *jy, A *jz,
-                                        A *nd) {
+// J (current) or n deposit of one particle at cell (k,l,m), offsets
+// (ox,oy,oz), into the workgroup's LDS window
+__device__ __forceinline__ void deposit_lds(const WinS &w, const Walk &wk, char *acc,
+                                            bool current, int k, int l, int m,
+                                            float ox, float oy, float oz, float vx,
+                                            float vy, float vz, const Push32 &kp) {
   const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
-  if (lds_ok) {
-    const int sl = 8 * w.WZ, sk = 8 * w.WY * w.WZ;
-    const int c0 = (((k - w.k0) * w.WY + (l - w.l0)) * w.WZ + (m - w.m0)) * 8;
-    const int hx = X.bh != k ? sk : 0, hy = Y.bh != l ? sl : 0, hz = Z.bh != m ? 8 : 0;
-    if (current) {
-      const float qs = kp.qw * kp.fx;
-      lds_dep8(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, X.i0, X.i1, Y.h0,
-               Y.h1, Z.h0, Z.h1);
-      lds_dep8(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Y.i0,
-               Y.i1, Z.h0, Z.h1);
-      lds_dep8(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, X.h0, X.h1, Y.h0,
-               Y.h1, Z.i0, Z.i1);
-    } else {
-      lds_dep8(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, X.h0,
-               X.h1, Y.h0, Y.h1, Z.h0, Z.h1);
-    }
+  const int sl = 8 * w.WZ, sk = 8 * w.WY * w.WZ;
+  const int c0 = (((k - w.k0) * w.WY + (l - w.l0)) * w.WZ + (m - w.m0)) * 8;
+  const int hx = X.bh != k ? sk : 0, hy = Y.bh != l ? sl : 0, hz = Z.bh != m ? 8 : 0;
+  if (current) {
+    const float qs = kp.qw * kp.fx;
+    lds_dep8(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, X.i0, X.i1, Y.h0, Y.h1,
+             Z.h0, Z.h1);
+    lds_dep8(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Y.i0, Y.i1,
+             Z.h0, Z.h1);
+    lds_dep8(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, X.h0, X.h1, Y.h0, Y.h1,
+             Z.i0, Z.i1);
   } else {
-    const float s = DET ? kp.fx : 1.0f;
-    if (current) {
-      glob_dep8(jx, g, X.bi, Y.bh, Z.bh, kp.qw * vx * s, X.i0, X.i1, Y.h0, Y.h1,
-                Z.h0, Z.h1);
-      glob_dep8(jy, g, X.bh, Y.bi, Z.bh, kp.qw * vy * s, X.h0, X.h1, Y.i0, Y.i1,
-                Z.h0, Z.h1);
-      glob_dep8(jz, g, X.bh, Y.bh, Z.bi, kp.qw * vz * s, X.h0, X.h1, Y.h0, Y.h1,
-                Z.i0, Z.i1);
-    } else {
-      glob_dep8(nd, g, X.bh, Y.bh, Z.bh, kp.w * s, X.h0, X.h1, Y.h0, Y.h1, Z.h0,
-                Z.h1);
-    }
+    lds_dep8(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, X.h0, X.h1,
+             Y.h0, Y.h1, Z.h0, Z.h1);
   }
+}
+
+// lanes of a wave holding more out-of-window particles than this deposit
+// their own particle each
+constexpr int kCoopMax = 8;
+
+// Deposit of the particles outside the window (pend), straight into the
+// global planes.  They are few and scattered over the waves; a lane that
+// issued its 8 (J: 24) atomics alone would hold the rest of its wave idle.
+// Instead the wave takes them one at a time: the particle's cell, offsets
+// and charges (q0..q2: J components, or q0: density) are read from its lane
+// and each active lane adds one node of one component.  Must be reached by
+// every active lane of the wave.  q carries the fixed-point scale in
+// deterministic mode only.
+template <typename A>
+__device__ __forceinline__ void glob_coop(const Geom &g, bool pend, bool current,
+                                          int k, int l, int m, float ox, float oy,
+                                          float oz, float q0, float q1, float q2,
+                                          A *jx, A *jy, A *jz, A *nd) {
+  u64 todo = __builtin_amdgcn_ballot_w64(pend);
+  if (todo == 0) return;
+  if (__builtin_popcountll(todo) > kCoopMax) {
+    if (pend) {
+      const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+      if (current) {
+        glob_dep8(jx, g, X.bi, Y.bh, Z.bh, q0, X.i0, X.i1, Y.h0, Y.h1, Z.h0, Z.h1);
+        glob_dep8(jy, g, X.bh, Y.bi, Z.bh, q1, X.h0, X.h1, Y.i0, Y.i1, Z.h0, Z.h1);
+        glob_dep8(jz, g, X.bh, Y.bh, Z.bi, q2, X.h0, X.h1, Y.h0, Y.h1, Z.i0, Z.i1);
+      } else {
+        glob_dep8(nd, g, X.bh, Y.bh, Z.bh, q0, X.h0, X.h1, Y.h0, Y.h1, Z.h0, Z.h1);
+      }
+    }
+    return;
+  }
+  const u64 ex = __builtin_amdgcn_read_exec();
+  const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+      (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
+  const int nact = __builtin_popcountll(ex);
+  const int nn = current ? 24 : 8;
+  const int64_t dl = g.SZ, dk = (int64_t)g.SY * g.SZ;
+  do {
+    const int src = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const int sk = __builtin_amdgcn_readlane(k, src);
+    const int sl = __builtin_amdgcn_readlane(l, src);
+    const int sm = __builtin_amdgcn_readlane(m, src);
+    const float sx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ox), src));
+    const float sy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(oy), src));
+    const float sz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(oz), src));
+    const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q0), src));
+    float s1 = 0.0f, s2 = 0.0f;
+    if (current) {
+      s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q1), src));
+      s2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q2), src));
+    }
+    const AxisW32 X = axis_w(sk, sx), Y = axis_w(sl, sy), Z = axis_w(sm, sz);
+    for (int t = rank; t < nn; t += nact) {
+      // node t: component t >> 3 (3: density), corner bits (x, y, z) of t & 7;
+      // the component's own axis takes the integer-node weights
+      const int c = current ? t >> 3 : 3;
+      const bool bx = t & 4, by = t & 2, bz = t & 1;
+      const float wx = c == 0 ? (bx ? X.i1 : X.i0) : (bx ? X.h1 : X.h0);
+      const float wy = c == 1 ? (by ? Y.i1 : Y.i0) : (by ? Y.h1 : Y.h0);
+      const float wz = c == 2 ? (bz ? Z.i1 : Z.i0) : (bz ? Z.h1 : Z.h0);
+      const int kx = c == 0 ? X.bi : X.bh, ly = c == 1 ? Y.bi : Y.bh,
+                mz = c == 2 ? Z.bi : Z.bh;
+      const float q = c == 1 ? s1 : c == 2 ? s2 : s0;
+      A *dst = c == 0 ? jx : c == 1 ? jy : c == 2 ? jz : nd;
+      const int64_t i = g.idx(kx, ly, mz) + (bx ? dk : 0) + (by ? dl : 0) + (bz ? 1 : 0);
+      glob_add(dst + i, (q * wx) * (wy * wz));
+    }
+  } while (todo);
 }
 
 // One particle of the fused pass.  PUSH_J: hot loop A (:731-812) - ballistic
@@ -403,6 +457,10 @@ __device__ __forceinline__ void particle_step(
   int k, l, m;
   unpack(cl, k, l, m);
   const bool lds_ok = in_win_s(w, k, l, m);
+  // fixed-point scale of the global path; in deterministic mode both paths
+  // form each node's contribution with the same products in the same order,
+  // so the sums do not depend on which particles the windows hold
+  const float fs = DET ? kp.fx : 1.0f;
   const auto inside = [&]() {
     return (unsigned)(k - 1) <= (unsigned)(g.SX - 3) &&
            (unsigned)(l - 1) <= (unsigned)(g.SY - 3) &&
@@ -423,17 +481,22 @@ __device__ __forceinline__ void particle_step(
     drift_axis(k, ox, vx, kp.hdt);
     drift_axis(l, oy, vy, kp.hdt);
     drift_axis(m, oz, vz, kp.hdt);
-    if (inside())
-      deposit<DET>(g, w, wk, acc, lds_ok, true, k, l, m, ox, oy, oz, vx, vy, vz,
-                   kp, jx, jy, jz, nd);
+    const bool dep = inside();
+    if (dep && lds_ok)
+      deposit_lds(w, wk, acc, true, k, l, m, ox, oy, oz, vx, vy, vz, kp);
+    const float qs = kp.qw * fs;
+    glob_coop(g, dep && !lds_ok, true, k, l, m, ox, oy, oz, qs * vx, qs * vy,
+              qs * vz, jx, jy, jz, nd);
   }
   if (DRIFT_N) {
     drift_axis(k, ox, vx, kp.hdt);
     drift_axis(l, oy, vy, kp.hdt);
     drift_axis(m, oz, vz, kp.hdt);
-    if (inside())
-      deposit<DET>(g, w, wk, acc, lds_ok, false, k, l, m, ox, oy, oz, vx, vy, vz,
-                   kp, jx, jy, jz, nd);
+    const bool dep = inside();
+    if (dep && lds_ok)
+      deposit_lds(w, wk, acc, false, k, l, m, ox, oy, oz, vx, vy, vz, kp);
+    glob_coop(g, dep && !lds_ok, false, k, l, m, ox, oy, oz, kp.w * fs, 0.0f, 0.0f,
+              jx, jy, jz, nd);
   }
   int32_t out = pack(k, l, m);
   if (BOUNDARY) {
@@ -471,8 +534,7 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
   __shared__ int64_t s_range[2];
   const Geom &g = G.g;
   const int tid = threadIdx.x;
-  const int nb = gridDim.x;
-  const int bid = xcd_remap(blockIdx.x, nb);
+  const int bid = xcd_remap(blockIdx.x, tr.ntiles, (int)gridDim.x);
 
   if (tid == 0) {
     int64_t s, e;
@@ -837,19 +899,38 @@ __global__ void k_load32(Geom g, Particles32 P, int64_t n, int ppc, double vth,
 }
 
 // add_*_flat refill (:6042-6104): same batch indexing as the FP64 form
+// Inflow batch (add_*_flat, :5993-6817): particle b of the reference order
+// (face, k, l, two z cells, ppc) is drawn from its own Philox counter, so
+// the batch can be stored in any order.  One thread per stored slot; the
+// slots take each face slab in blocks of 4 x 4 x 2 cells (ppc consecutive
+// particles per cell) when NX and NY are multiples of 4, so that a tail
+// chunk of the next passes covers a compact box of cells.
 __global__ void k_refill32(Geom g, Particles32 P, int64_t base, int64_t count,
                            int ppc, double vth, double mu0, double mu1,
                            double mu2, uint64_t seed, int sp, uint32_t event) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= count) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
   const int64_t per_face = (int64_t)g.NX * g.NY * 2 * ppc;
-  const int face = (int)(b / per_face);
-  const int64_t r = b - face * per_face;
-  const int m = (int)((r / ppc) % 2);
-  const int l = (int)((r / (2 * ppc)) % g.NY);
-  const int k = (int)(r / ((int64_t)2 * ppc * g.NY));
+  const int face = (int)(t / per_face);
+  const int64_t r = t - face * per_face;
+  const int i = (int)(r % ppc);
+  const int64_t c = r / ppc;  // cell of the slot order
+  int k, l, m;
+  if (g.NX % 4 == 0 && g.NY % 4 == 0) {
+    const int w = (int)(c % 32);
+    const int64_t blk = c / 32;
+    const int nby = g.NY / 4;
+    m = w & 1;
+    l = (int)(blk % nby) * 4 + ((w >> 1) & 3);
+    k = (int)(blk / nby) * 4 + (w >> 3);
+  } else {
+    m = (int)(c % 2);
+    l = (int)((c / 2) % g.NY);
+    k = (int)(c / (2 * (int64_t)g.NY));
+  }
+  const int64_t b = face * per_face + (((int64_t)k * g.NY + l) * 2 + m) * ppc + i;
   const int mc = face == 0 ? m + 2 : m + g.NZ;
-  store_draw(P, base + b, k + 2, l + 2, mc,
+  store_draw(P, base + t, k + 2, l + 2, mc,
              draw_particle(seed, sp, event, (uint64_t)b), vth, mu0, mu1, mu2);
 }
 
@@ -1288,7 +1369,9 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
                      "fp32 particle pass: %lld stored particles exceed the "
                      "32-bit slot index", (long long)n);
   State32 *S = st(ctx);
-  const TileGeom tg = tile_geom(ctx);
+  TileGeom tg = tile_geom(ctx);
+  add_drift_shift(tg, ctx->cfg, ctx->drift_halves[sp]);
+  ctx->drift_halves[sp] += (do_push_j ? 1 : 0) + (do_drift_n ? 1 : 0);
   TileRun tr;
   tr.tile_start = ctx->tile_start[sp];
   tr.sorted_end = ctx->sorted_end[sp];
@@ -1406,6 +1489,7 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
 int sort(pic_ctx *ctx, int sp) {
   const Geom &g = ctx->geom;
   State32 *S = st(ctx);
+  ctx->drift_halves[sp] = 0;
   const TileGeom tg = tile_geom(ctx);
   const int nt = tg.ntiles();
   const int64_t n = ctx->n[sp];

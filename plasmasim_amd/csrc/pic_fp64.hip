@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) void k_particles_tiled(
   __shared__ Win sw;
   __shared__ int64_t s_range[2];
   const int tid = threadIdx.x;
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int bid = xcd_remap(blockIdx.x, tr.ntiles, (int)gridDim.x);
   const double dx = kc.dx;
   if (tid == 0) {
     int64_t s, e;

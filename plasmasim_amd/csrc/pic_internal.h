@@ -27,6 +27,7 @@ struct Geom {
 struct TileGeom {
   int TX, TY, TZ;     // cells per tile
   int NTX, NTY, NTZ;  // tiles per axis
+  int shift[3];       // cells by which the tiles' LDS windows are moved
   __host__ __device__ inline uint32_t key(int k, int l, int m) const {
     const uint32_t tile = ((uint32_t)(k / TX) * NTY + l / TY) * NTZ + m / TZ;
     const uint32_t in = ((uint32_t)(k % TX) * TY + l % TY) * TZ + m % TZ;
@@ -108,6 +109,7 @@ struct pic_ctx {
   // sort; [0, sorted_end) is tile-sorted, later slots are the inflow tail
   uint32_t *tile_start[2] = {nullptr, nullptr};   // ntiles + 1 per species
   int64_t sorted_end[2] = {0, 0};
+  int32_t drift_halves[2] = {0, 0};  // half-step drifts since the tile sort
 
   // state private to the numeric form (FP32: sigma copy, tile ranges,
   // deterministic accumulators)
@@ -194,6 +196,7 @@ inline TileGeom tile_geom(const pic_ctx *ctx) {
   t.NTX = c.nx / t.TX;
   t.NTY = c.ny / t.TY;
   t.NTZ = c.nz / t.TZ;
+  for (int a = 0; a < 3; ++a) t.shift[a] = 0;
   return t;
 }
 

@@ -17,6 +17,7 @@ namespace pic {
 
 constexpr int kMargin = 3;
 constexpr int kTailChunk = 2048;
+constexpr int kStray = 8;
 
 struct TileRun {
   const uint32_t *tile_start;  // ntiles + 1 entries
@@ -34,12 +35,29 @@ struct Win {
   bool on;
 };
 
-__device__ inline int xcd_remap(int b, int nb) {
-  // consecutive workgroups are dealt round-robin over the 8 XCDs; give each
-  // XCD a contiguous range of tiles so neighbouring tiles share its L2
-  const int q = nb / 8;
-  if (b >= 8 * q) return b;
-  return (b % 8) * q + b / 8;
+// Between sorts the particles move with the bulk drift (vdrift) and leave
+// their tile on its downstream side; those more than one cell out deposit
+// through global atomics.  The tiles' windows follow the drift: after
+// `halves` half-step drifts since the sort they are moved by the bulk
+// displacement rounded to whole cells.
+inline void add_drift_shift(TileGeom &t, const pic_config &c, int halves) {
+  for (int a = 0; a < 3; ++a)
+    t.shift[a] = (int)lround(c.vdrift[a] * 0.5 * c.dt * halves / c.dx);
+}
+
+__device__ inline int xcd_remap(int b, int nt, int nb) {
+  // Workgroups are dealt round-robin over the 8 XCDs in launch order.  Each
+  // XCD gets a contiguous range of the nt tiles (neighbouring tiles share its
+  // L2) and then an equal share of the nb - nt tail chunks, so the light
+  // chunk workgroups do not displace tiles onto the other XCDs.
+  const int nc = nb - nt;
+  const int qt = nt / 8, qc = nc / 8, q = qt + qc;
+  if (b >= 8 * q) {  // remainders of both kinds, in order
+    const int r = b - 8 * q, rt = nt - 8 * qt;
+    return r < rt ? 8 * qt + r : nt + 8 * qc + (r - rt);
+  }
+  const int x = b % 8, s = b / 8;
+  return s < qt ? x * qt + s : nt + x * qc + (s - qt);
 }
 
 // Range and window of workgroup bid.  cell_of(i, k, l, m) returns false for
@@ -54,11 +72,11 @@ __device__ inline void tile_setup(int bid, const TileRun &tr, const TileGeom &tg
     e = tr.tile_start[bid + 1];
     const int tz = bid % tg.NTZ, ty = (bid / tg.NTZ) % tg.NTY,
               tx = bid / (tg.NTZ * tg.NTY);
-    ak0 = 2 + tx * tg.TX;
+    ak0 = 2 + tx * tg.TX + tg.shift[0];
     ak1 = ak0 + tg.TX - 1;
-    al0 = 2 + ty * tg.TY;
+    al0 = 2 + ty * tg.TY + tg.shift[1];
     al1 = al0 + tg.TY - 1;
-    am0 = 2 + tz * tg.TZ;
+    am0 = 2 + tz * tg.TZ + tg.shift[2];
     am1 = am0 + tg.TZ - 1;
   } else {
     const int c = bid - tr.ntiles;
@@ -66,14 +84,27 @@ __device__ inline void tile_setup(int bid, const TileRun &tr, const TileGeom &tg
     e = s + kTailChunk < tr.n ? s + kTailChunk : tr.n;
     ak0 = al0 = am0 = 1 << 20;
     ak1 = al1 = am1 = -1;
+    // 8 samples; those more than kStray cells from a middle sample on some
+    // axis (a particle that wrapped across a periodic face, a straggler) are
+    // left out of the box and deposit globally
+    int ks[8], ls[8], ms[8];
+    bool ok[8];
     for (int j = 0; j < 8; ++j) {
       const int64_t i = s + (e - s - 1) * j / 7;
-      if (i < s || i >= e) continue;
-      int k, l, m;
-      if (!cell_of(i, k, l, m)) continue;
-      ak0 = min(ak0, k); ak1 = max(ak1, k);
-      al0 = min(al0, l); al1 = max(al1, l);
-      am0 = min(am0, m); am1 = max(am1, m);
+      ok[j] = i >= s && i < e && cell_of(i, ks[j], ls[j], ms[j]);
+    }
+    int r = -1;
+    for (int q = 0; q < 8 && r < 0; ++q) {
+      const int j = (q & 1) ? 3 - (q >> 1) : 4 + (q >> 1);  // 4 3 5 2 6 1 7 0
+      if (ok[j]) r = j;
+    }
+    for (int j = 0; j < 8 && r >= 0; ++j) {
+      if (!ok[j] || abs(ks[j] - ks[r]) > kStray || abs(ls[j] - ls[r]) > kStray ||
+          abs(ms[j] - ms[r]) > kStray)
+        continue;
+      ak0 = min(ak0, ks[j]); ak1 = max(ak1, ks[j]);
+      al0 = min(al0, ls[j]); al1 = max(al1, ls[j]);
+      am0 = min(am0, ms[j]); am1 = max(am1, ms[j]);
     }
   }
   w.on = ak1 >= ak0;

@@ -125,17 +125,18 @@ def test_fp32_resume_with_other_tiles_continues(tmp_path):
     a.step(3)
     b.step(3)
     # the sort in these steps orders each run by its own tiles: fields and
-    # counts compare directly, particles as sets; deposits reach the planes
-    # through other windows, so fields differ by rounding only
+    # counts compare directly, particles as sets.  Deposits reach the planes
+    # through other windows (or straight from the particles), and in
+    # deterministic mode each contribution is the same fixed-point integer on
+    # either path, so the fields are bit-identical
     for n in FIELDS:
-        e = rel_l2(a.download_field(n), b.download_field(n))
-        assert e <= 1e-6, (n, e)
+        assert np.array_equal(a.download_field(n), b.download_field(n)), n
     for sp in (0, 1):
         assert a.particle_count(sp) == b.particle_count(sp)
         pa, pb = a.download_particles(sp), b.download_particles(sp)
         pa = pa[np.lexsort(pa[:, ::-1].T)]
         pb = pb[np.lexsort(pb[:, ::-1].T)]
-        assert rel_l2(pa, pb) <= 1e-6, sp
+        assert np.array_equal(pa, pb), sp
 
 
 def test_rejected_checkpoints(tmp_path):
