@@ -5,6 +5,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import plasmasim_amd
 from plasmasim_amd import _lib
 
@@ -77,3 +79,10 @@ def test_standoff_formula():
         got = (c.mu0 * c.dipole_m ** 2 /
                (8 * math.pi ** 2 * n0 * c.mass[0] * V * V)) ** (1 / 6)
         assert abs(got - L) < 1e-9
+
+
+def test_autograd_bridge_checks_arguments_on_host():
+    import torch
+    from plasmasim_amd.autograd import thrust_loss
+    with pytest.raises(ValueError):
+        thrust_loss(None, torch.zeros(3, dtype=torch.float64), 4)

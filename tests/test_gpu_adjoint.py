@@ -77,3 +77,27 @@ def test_fp32_forward_gradient_near_fp64():
           "grad rel", rel)
     assert abs(l32 - l64) <= 1e-3 * abs(l64)
     assert np.all(rel <= 2e-2), rel
+
+
+def test_torch_autograd_bridge():
+    """thrust_loss composes with torch: d(3 L(p) + |p|^2)/dp = 3 (adjoint
+    gradient) + 2 p; a second forward before the first loss is
+    back-propagated is refused."""
+    import torch
+    from plasmasim_amd.autograd import thrust_loss
+    c = cfg()
+    _, g = gpu_grad(c)
+    s = ps.Simulation(c)
+    p = torch.tensor(params(c), dtype=torch.float64, requires_grad=True)
+    L = thrust_loss(s, p, NSTEPS, 2, R, NQ)
+    (3.0 * L + (p * p).sum()).backward()
+    want = 3.0 * g + 2.0 * params(c)
+    assert np.allclose(p.grad.numpy(), want, rtol=1e-9, atol=0), (p.grad, want)
+    L1 = thrust_loss(s, p, NSTEPS, 2, R, NQ)
+    L2 = thrust_loss(s, p, NSTEPS, 2, R, NQ)
+    with pytest.raises(RuntimeError):
+        L1.backward()
+    p.grad = None
+    L2.backward()
+    assert np.allclose(p.grad.numpy(), g, rtol=1e-9, atol=0)
+    s.close()
