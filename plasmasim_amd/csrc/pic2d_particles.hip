@@ -374,7 +374,27 @@ struct K32 {
   float blo[2], bhi[2];  // ballistic bounds in cell units
   float fx;              // fixed-point units per unit value
   double fx_inv;
+  uint32_t cmag;         // cell = k SZ + m: k = mulhi(cell, cmag) >> csh
+  int csh;
 };
+
+// Exact k = n / SZ for 0 <= n < 2^31: with l = ceil(log2 SZ) and
+// M = ceil(2^(31+l) / SZ) (below 2^32), n M / 2^(31+l) exceeds n / SZ by
+// less than 1 / SZ, which never reaches the next integer.
+inline void cell_magic(int SZ, uint32_t &mag, int &sh) {
+  int l = 0;
+  while ((1 << l) < SZ) ++l;
+  if (l == 0) l = 1;
+  const unsigned __int128 one = 1;
+  mag = (uint32_t)(((one << (31 + l)) + (unsigned)SZ - 1) / (unsigned)SZ);
+  sh = l - 1;
+}
+
+__device__ inline void split_cell(const Geom2 &g, const K32 &kp, int32_t cl, int &k,
+                                  int &m) {
+  k = (int)(__umulhi((uint32_t)cl, kp.cmag) >> kp.csh);
+  m = cl - k * g.SZ;
+}
 
 struct F32Fields {
   const float *ex, *ey, *ez, *bx, *by, *bz;
@@ -401,20 +421,31 @@ __device__ inline Ax axis2(int k, float o) {
   return a;
 }
 
-__device__ inline float lerp4(const float *__restrict__ F, int64_t i, int64_t dk,
-                              float a0, float a1, float c0, float c1) {
-  return a0 * (c0 * F[i] + c1 * F[i + 1]) + a1 * (c0 * F[i + dk] + c1 * F[i + dk + 1]);
+__device__ __forceinline__ const float *at(const float *base, uint32_t byte) {
+  return (const float *)((const char *)base + byte);
 }
 
-__device__ inline void gather32(const Geom2 &g, const F32Fields &F, const Ax &X,
-                                const Ax &Z, float f[6]) {
-  const int64_t dk = g.SZ;
-  f[0] = lerp4(F.ex, g.idx(X.bi, Z.bh), dk, X.i0, X.i1, Z.h0, Z.h1);
-  f[1] = lerp4(F.ey, g.idx(X.bh, Z.bh), dk, X.h0, X.h1, Z.h0, Z.h1);
-  f[2] = lerp4(F.ez, g.idx(X.bh, Z.bi), dk, X.h0, X.h1, Z.i0, Z.i1);
-  f[3] = lerp4(F.bx, g.idx(X.bh, Z.bi), dk, X.h0, X.h1, Z.i0, Z.i1);
-  f[4] = lerp4(F.by, g.idx(X.bi, Z.bi), dk, X.i0, X.i1, Z.i0, Z.i1);
-  f[5] = lerp4(F.bz, g.idx(X.bi, Z.bh), dk, X.i0, X.i1, Z.h0, Z.h1);
+// bilinear interpolation around the byte offset o of the stencil base (two
+// adjacent m values per row; sk: byte stride of k)
+__device__ __forceinline__ float lerp4o(const float *F, uint32_t o, uint32_t sk,
+                                        float a0, float a1, float c0, float c1) {
+  const float *p0 = at(F, o), *p1 = at(F, o + sk);
+  return a0 * (c0 * p0[0] + c1 * p0[1]) + a1 * (c0 * p1[0] + c1 * p1[1]);
+}
+
+// shape_func_*3 (:1948-2011): the six staggered components from uniform plane
+// bases and 32-bit byte offsets (the planes are below 2^31 bytes); o is the
+// byte offset of the particle's cell, hk / hm the byte shifts of the
+// half-node bases
+__device__ inline void gather32(const F32Fields &F, uint32_t o, uint32_t sk,
+                                uint32_t hk, uint32_t hm, const Ax &X, const Ax &Z,
+                                float f[6]) {
+  f[0] = lerp4o(F.ex, o - hm, sk, X.i0, X.i1, Z.h0, Z.h1);
+  f[1] = lerp4o(F.ey, o - hk - hm, sk, X.h0, X.h1, Z.h0, Z.h1);
+  f[2] = lerp4o(F.ez, o - hk, sk, X.h0, X.h1, Z.i0, Z.i1);
+  f[3] = lerp4o(F.bx, o - hk, sk, X.h0, X.h1, Z.i0, Z.i1);
+  f[4] = lerp4o(F.by, o, sk, X.i0, X.i1, Z.i0, Z.i1);
+  f[5] = lerp4o(F.bz, o - hm, sk, X.i0, X.i1, Z.h0, Z.h1);
 }
 
 __device__ inline void boris32(const K32 &p, const float f[6], float &vx,
@@ -453,77 +484,195 @@ __device__ inline void drift32(int &k, float &o, float v, float hdt) {
   }
 }
 
-__device__ inline u64 to_fx(float v, float s) {
-  return (u64)(long long)(int)rintf(v * s);
+// Fixed-point contribution: the scale is folded into the deposit factor;
+// the scaled value is rounded to the nearest unit and sign-extended.
+__device__ inline u64 to_fx(float v) { return (u64)(long long)(int)rintf(v); }
+
+// The lane's walk over the 4 nodes of a stencil.  Lanes of a wave mostly
+// hold particles of the same cell (the data are cell sorted; at 100 per cell
+// a wave's 256 particles span about 3 cells) and their LDS atomics to one
+// node serialise.  Lane j visits node n ^ (j & 3) at step n, which spreads
+// one instruction's lanes over the four nodes, and accumulates into copy
+// (j >> 2) mod ncopy of the window (ncopy copies of WN nodes side by side
+// in each plane, summed at the flush).
+struct Walk2 {
+  bool mx, mz;  // axes walked downwards
+  int off[4];   // node offsets of the visits from the stencil base
+};
+
+__device__ inline Walk2 make_walk2(int WZ, int WN, int ncopy) {
+  Walk2 k;
+  const int sw = threadIdx.x & 3;
+  const int copy = ((int)threadIdx.x >> 2) & (ncopy - 1);
+  k.mx = sw & 1;
+  k.mz = sw & 2;
+  for (int n = 0; n < 4; ++n) {
+    const int id = n ^ sw;
+    k.off[n] = copy * WN + ((id & 1) ? WZ : 0) + (id >> 1);
+  }
+  return k;
 }
 
-__device__ inline void dep4_32(u64 *lds, float *glob, const Win2 &w,
-                               const Geom2 &g, int k, int m, float a0, float a1,
-                               float c0, float c1, float q, float s) {
-  const float qa0 = q * a0, qa1 = q * a1;
-  if (k < 0 || m < 0 || k + 1 >= g.SX || m + 1 >= g.SZ) return;
-  if (w.holds(k, m)) {
-    const int i = (k - w.k0) * w.WZ + (m - w.m0);
-    const int sw = threadIdx.x & 3;
+// q a_i c_j at the 4 window nodes of a stencil based at window index i; q
+// carries the fixed-point scale
+__device__ __forceinline__ void lds_dep4(u64 *lds, int i, const Walk2 &wk, float q,
+                                         float a0, float a1, float c0, float c1) {
+  const float A0 = wk.mx ? a1 : a0, A1 = wk.mx ? a0 : a1;
+  const float C0 = wk.mz ? c1 : c0, C1 = wk.mz ? c0 : c1;
+  const float qa0 = q * A0, qa1 = q * A1;
+  const float v[4] = {qa0 * C0, qa1 * C0, qa0 * C1, qa1 * C1};
 #pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const int id = n ^ sw;
-      const float v = ((id & 1) ? qa1 : qa0) * ((id & 2) ? c1 : c0);
-      __hip_atomic_fetch_add((PIC_LDS u64 *)(lds + i + ((id & 1) ? w.WZ : 0) + (id >> 1)),
-                             to_fx(v, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-  } else {
-    const int64_t i = g.idx(k, m);
-#pragma unroll
-    for (int n = 0; n < 4; ++n) {
-      const float v = ((n & 1) ? qa1 : qa0) * ((n & 2) ? c1 : c0);
-      __hip_atomic_fetch_add((PIC_GLB float *)(glob + i + ((n & 1) ? g.SZ : 0) + (n >> 1)),
-                             v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  for (int n = 0; n < 4; ++n)
+    __hip_atomic_fetch_add((PIC_LDS u64 *)(lds + i + wk.off[n]), to_fx(v[n]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 struct Dep32 {
   float *jx, *jy, *jz, *n;
 };
 
+__device__ inline void glb_add32(float *p, float v) {
+  __hip_atomic_fetch_add((PIC_GLB float *)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one lane's stencil straight into a global plane
+__device__ __noinline__ void glob_dep4(float *glob, const Geom2 &g, int k, int m,
+                                       float q, float a0, float a1, float c0,
+                                       float c1) {
+  const int64_t i = g.idx(k, m);
+  const float qa0 = q * a0, qa1 = q * a1;
+  glb_add32(glob + i, qa0 * c0);
+  glb_add32(glob + i + g.SZ, qa1 * c0);
+  glb_add32(glob + i + 1, qa0 * c1);
+  glb_add32(glob + i + g.SZ + 1, qa1 * c1);
+}
+
+constexpr int kCoopMax = 8;
+
+// Deposit of the particles outside the window (pend) into the global planes:
+// the wave takes them one at a time, each active lane adding one node of one
+// component (J: 12 nodes, n: 4); waves with more than kCoopMax such
+// particles deposit lane by lane.  Reached by every active lane of the wave.
+__device__ __forceinline__ void glob_coop2(const Geom2 &g, const Dep32 &D, bool pend,
+                                           bool current, int k, int m, float ox,
+                                           float oz, float q0, float q1, float q2) {
+  u64 todo = __builtin_amdgcn_ballot_w64(pend);
+  if (todo == 0) return;
+  if (__builtin_popcountll(todo) > kCoopMax) {
+    if (pend) {
+      const Ax X = axis2(k, ox), Z = axis2(m, oz);
+      if (current) {
+        glob_dep4(D.jx, g, X.bi, Z.bh, q0, X.i0, X.i1, Z.h0, Z.h1);
+        glob_dep4(D.jy, g, X.bh, Z.bh, q1, X.h0, X.h1, Z.h0, Z.h1);
+        glob_dep4(D.jz, g, X.bh, Z.bi, q2, X.h0, X.h1, Z.i0, Z.i1);
+      } else {
+        glob_dep4(D.n, g, X.bh, Z.bh, q0, X.h0, X.h1, Z.h0, Z.h1);
+      }
+    }
+    return;
+  }
+  const u64 ex = __builtin_amdgcn_read_exec();
+  const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+      (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
+  const int nact = __builtin_popcountll(ex);
+  const int nn = current ? 12 : 4;
+  do {
+    const int src = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const int sk = __builtin_amdgcn_readlane(k, src);
+    const int sm = __builtin_amdgcn_readlane(m, src);
+    const float sx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ox), src));
+    const float sz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(oz), src));
+    const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q0), src));
+    float s1 = 0.0f, s2 = 0.0f;
+    if (current) {
+      s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q1), src));
+      s2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q2), src));
+    }
+    const Ax X = axis2(sk, sx), Z = axis2(sm, sz);
+    for (int t = rank; t < nn; t += nact) {
+      // node t: component t >> 2 (3: density), corner bits (x, z) of t & 3;
+      // x takes the integer-node weights for Jx, z for Jz
+      const int c = current ? t >> 2 : 3;
+      const bool bx = t & 1, bz = t & 2;
+      const float wx = c == 0 ? (bx ? X.i1 : X.i0) : (bx ? X.h1 : X.h0);
+      const float wz = c == 2 ? (bz ? Z.i1 : Z.i0) : (bz ? Z.h1 : Z.h0);
+      const int kx = (c == 0 ? X.bi : X.bh) + (bx ? 1 : 0);
+      const int mz = (c == 2 ? Z.bi : Z.bh) + (bz ? 1 : 0);
+      const float q = c == 1 ? s1 : c == 2 ? s2 : s0;
+      float *dst = c == 0 ? D.jx : c == 1 ? D.jy : c == 2 ? D.jz : D.n;
+      glb_add32(dst + g.idx(kx, mz), (q * wx) * wz);
+    }
+  } while (todo);
+}
+
+// the stencils of a particle at cell (k, m) (bases k-1..k, m-1..m) lie in the
+// window
+__device__ inline bool in_win2(const Win2 &w, int k, int m) {
+  return w.on && (unsigned)(k - 1 - w.k0) <= (unsigned)(w.WX - 3) &&
+         (unsigned)(m - 1 - w.m0) <= (unsigned)(w.WZ - 3);
+}
+
 template <bool PUSH_J, bool DRIFT, bool DEP_N, bool BOUNDARY>
 __device__ __forceinline__ void step32(const Geom2 &g, const F32Fields &F,
-                                       const K32 &kp, const Win2 &w, u64 *acc,
+                                       const K32 &kp, const Win2 &w,
+                                       const Walk2 &wk, u64 *acc,
                                        const Dep32 &D, int32_t &cl, float &ox,
                                        float &oz, float &vx, float &vy,
                                        float &vz) {
   if (cl < 0) return;
-  int k = cl / g.SZ, m = cl - (cl / g.SZ) * g.SZ;
+  int k, m;
+  split_cell(g, kp, cl, k, m);
+  // deposits only from cells whose stencils lie inside the array
+  const auto dep_gate = [&]() {
+    return (unsigned)(k - 1) <= (unsigned)(g.NX + 1) &&
+           (unsigned)(m - 2) <= (unsigned)(g.NZ - 1);
+  };
   if (PUSH_J) {
     const float xf = (float)k + ox, zf = (float)m + oz;
-    const bool safe = k >= 1 && k <= g.SX - 2 && m >= 1 && m <= g.SZ - 2;
+    const bool safe = (unsigned)(k - 1) <= (unsigned)(g.SX - 3) &&
+                      (unsigned)(m - 1) <= (unsigned)(g.SZ - 3);
     if (safe && !(zf < kp.blo[1] || zf > kp.bhi[1] || xf < kp.blo[0] ||
                   xf > kp.bhi[0])) {
+      const uint32_t sk = 4u * (uint32_t)g.SZ;
+      const uint32_t o = ((uint32_t)k * (uint32_t)g.SZ + (uint32_t)m) * 4u;
+      const Ax X = axis2(k, ox), Z = axis2(m, oz);
       float f[6];
-      gather32(g, F, axis2(k, ox), axis2(m, oz), f);
+      gather32(F, o, sk, X.bh != k ? sk : 0u, Z.bh != m ? 4u : 0u, X, Z, f);
       boris32(kp, f, vx, vy, vz);
     }
     drift32(k, ox, vx, kp.hdt);
     drift32(m, oz, vz, kp.hdt);
-    if (k >= 1 && k <= g.NX + 2 && m >= 2 && m <= g.NZ + 1) {
+    const bool dep = dep_gate();
+    const bool lds_ok = in_win2(w, k, m);
+    const float qs = kp.qw * kp.fx;
+    if (dep && lds_ok) {
       const Ax X = axis2(k, ox), Z = axis2(m, oz);
-      dep4_32(acc + 1 * kWin2, D.jy, w, g, X.bh, Z.bh, X.h0, X.h1, Z.h0, Z.h1,
-              kp.qw * vy, kp.fx);
-      dep4_32(acc + 0 * kWin2, D.jx, w, g, X.bi, Z.bh, X.i0, X.i1, Z.h0, Z.h1,
-              kp.qw * vx, kp.fx);
-      dep4_32(acc + 2 * kWin2, D.jz, w, g, X.bh, Z.bi, X.h0, X.h1, Z.i0, Z.i1,
-              kp.qw * vz, kp.fx);
+      const int c0 = (k - w.k0) * w.WZ + (m - w.m0);
+      const int hx = X.bh != k ? w.WZ : 0, hz = Z.bh != m ? 1 : 0;
+      lds_dep4(acc + 0 * kWin2, c0 - hz, wk, qs * vx, X.i0, X.i1, Z.h0, Z.h1);
+      lds_dep4(acc + 1 * kWin2, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Z.h0, Z.h1);
+      lds_dep4(acc + 2 * kWin2, c0 - hx, wk, qs * vz, X.h0, X.h1, Z.i0, Z.i1);
     }
+    glob_coop2(g, D, dep && !lds_ok, true, k, m, ox, oz, kp.qw * vx, kp.qw * vy,
+               kp.qw * vz);
   }
   if (DRIFT) {
     drift32(k, ox, vx, kp.hdt);
     drift32(m, oz, vz, kp.hdt);
   }
-  if (DEP_N && k >= 1 && k <= g.NX + 2 && m >= 2 && m <= g.NZ + 1) {
-    const Ax X = axis2(k, ox), Z = axis2(m, oz);
-    dep4_32(acc + 3 * kWin2, D.n, w, g, X.bh, Z.bh, X.h0, X.h1, Z.h0, Z.h1,
-            kp.w, kp.fx);
+  if (DEP_N) {
+    const bool dep = dep_gate();
+    const bool lds_ok = in_win2(w, k, m);
+    if (dep && lds_ok) {
+      const Ax X = axis2(k, ox), Z = axis2(m, oz);
+      const int c0 = (k - w.k0) * w.WZ + (m - w.m0);
+      const int hx = X.bh != k ? w.WZ : 0, hz = Z.bh != m ? 1 : 0;
+      lds_dep4(acc + 3 * kWin2, c0 - hx - hz, wk, kp.w * kp.fx, X.h0, X.h1, Z.h0,
+               Z.h1);
+    }
+    glob_coop2(g, D, dep && !lds_ok, false, k, m, ox, oz, kp.w, 0.0f, 0.0f);
   }
   bool dead = k < 0 || m < 0 || k >= g.SX || m >= g.SZ;
   if (BOUNDARY && !dead) {
@@ -554,8 +703,7 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
            [&](int64_t i, int &k, int &m) {
              const int32_t c = P.cell[i];
              if (c < 0) return false;
-             k = c / g.SZ;
-             m = c - k * g.SZ;
+             split_cell(g, kp, c, k, m);
              return true;
            },
            s, e, w);
@@ -569,8 +717,11 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
   if (s >= e) return;
   const Win2 w = sw;
   const int WN = w.on ? w.WX * w.WZ : 0;
-  for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin2 + i % WN] = 0;
+  const int ncopy = 4 * WN <= kWin2 ? 4 : 2 * WN <= kWin2 ? 2 : 1;
+  const int WC = ncopy * WN;  // nodes in use per plane
+  for (int i = tid; i < 4 * WC; i += blockDim.x) acc[(i / WC) * kWin2 + i % WC] = 0;
   __syncthreads();
+  const Walk2 wk = make_walk2(w.WZ, WN, ncopy);
   // four consecutive particles per thread with 16-byte accesses; slots of a
   // boundary quad outside [s, e) belong to a neighbouring range
   const int64_t q0 = s >> 2, q1 = (e + 3) >> 2;
@@ -591,7 +742,7 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
     for (int j = 0; j < 4; ++j) {
       const int64_t i = i0 + j;
       if (i < s || i >= e) continue;
-      step32<PUSH_J, DRIFT, DEP_N, BOUNDARY>(g, F, kp, w, acc, D, cs[j], xs[j],
+      step32<PUSH_J, DRIFT, DEP_N, BOUNDARY>(g, F, kp, w, wk, acc, D, cs[j], xs[j],
                                              zs[j], us[j], vs[j], ws[j]);
     }
     if (full) {
@@ -627,13 +778,12 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
     if (c == 3 && !DEP_N) continue;
     float *dst = c == 0 ? D.jx : c == 1 ? D.jy : c == 2 ? D.jz : D.n;
     for (int j = tid; j < WN; j += blockDim.x) {
-      const u64 v = acc[c * kWin2 + j];
+      u64 v = 0;
+      for (int cp = 0; cp < ncopy; ++cp) v += acc[c * kWin2 + cp * WN + j];
       if (v == 0) continue;
       const int kk = w.k0 + j / w.WZ, mm = w.m0 + j % w.WZ;
       if (kk < 0 || mm < 0 || kk >= g.SX || mm >= g.SZ) continue;
-      __hip_atomic_fetch_add((PIC_GLB float *)(dst + g.idx(kk, mm)),
-                             (float)((double)(long long)v * inv_s),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      glb_add32(dst + g.idx(kk, mm), (float)((double)(long long)v * inv_s));
     }
   }
 }
@@ -722,6 +872,7 @@ int push(pic2d_ctx *ctx, int sp, int push_j, int drift_n, int boundary,
     }
     kp.fx = (float)ctx->fx_scale;
     kp.fx_inv = 1.0 / ctx->fx_scale;
+    cell_magic(ctx->g.SZ, kp.cmag, kp.csh);
 #define L32(a, b, c, d) \
   k_pass32<a, b, c, d><<<nb, 256, 0, st>>>(ctx->g, ctx->tg, F, P, tr, kp, D)
     switch (sel) {
