@@ -296,3 +296,28 @@ def test_out_of_window_particles_match_fp64(deposit):
     if len(runs) == 2:
         for n in FIELDS + J:
             assert np.array_equal(runs[0].download_field(n), runs[1].download_field(n)), n
+
+
+@pytest.mark.parametrize("preset", [1, 2])
+def test_inflow_batch_matches_fp64_as_set(preset):
+    """The FP32 form stores an inflow batch in blocks of 4 x 4 x 2 cells; each
+    particle is still drawn from the counter of its reference-order index, so
+    the batch is the FP64 form's batch as a set: cells exact, offsets and
+    velocities within fp32 rounding."""
+    out = {}
+    for prec in (ps.PIC_FP32, ps.PIC_FP64):
+        c = cfg(preset, precision=prec)
+        if preset == 2:
+            c.nx = c.ny = 32  # a 32 x 32 face of the 128-cell-deep workload
+        s = ps.Simulation(c)
+        s.inflow()
+        out[prec] = [s.download_particles(sp) for sp in (0, 1)]
+        s.close()
+    for sp in (0, 1):
+        a, b = out[ps.PIC_FP32][sp], out[ps.PIC_FP64][sp]
+        assert a.shape == b.shape and len(a) > 0
+        a = a[np.lexsort(np.column_stack([np.floor(a[:, :3]), a[:, 3:]])[:, ::-1].T)]
+        b = b[np.lexsort(np.column_stack([np.floor(b[:, :3]), b[:, 3:]])[:, ::-1].T)]
+        assert np.array_equal(np.floor(a[:, :3]), np.floor(b[:, :3])), sp
+        assert np.abs(a[:, :3] - b[:, :3]).max() < 1.2e-7 * 2
+        assert np.abs(a[:, 3:] - b[:, 3:]).max() <= 6e-8 * np.abs(b[:, 3:]).max()
