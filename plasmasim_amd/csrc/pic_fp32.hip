@@ -501,22 +501,15 @@ __device__ __forceinline__ void particle_step(
   int32_t out = pack(k, l, m);
   if (BOUNDARY) {
     // sr_particle (:2041-2531): z must stay in [2, NZ+2) cells; x and y
-    // wrap periodically (shift by the box length) or drop
-    bool dead = !(m >= 2 && m < g.NZ + 2) || k < 0 || l < 0 || k >= g.SX ||
-                l >= g.SY;
-    if (!dead) {
-      if (k >= g.NX + 2) {
-        if (kp.periodic[0]) k -= g.NX; else dead = true;
-      } else if (k < 2) {
-        if (kp.periodic[0]) k += g.NX; else dead = true;
-      }
-      if (l >= g.NY + 2) {
-        if (kp.periodic[1]) l -= g.NY; else dead = true;
-      } else if (l < 2) {
-        if (kp.periodic[1]) l += g.NY; else dead = true;
-      }
-    }
-    out = dead ? -1 : pack(k, l, m);
+    // wrap periodically (shift by the box length) or drop.  Written with
+    // selects: the lanes of a wave take different branches here.
+    const bool inside_zs = (unsigned)(m - 2) < (unsigned)g.NZ &&
+                           (unsigned)k < (unsigned)g.SX && (unsigned)l < (unsigned)g.SY;
+    const int dk = k >= g.NX + 2 ? -g.NX : k < 2 ? g.NX : 0;
+    const int dl = l >= g.NY + 2 ? -g.NY : l < 2 ? g.NY : 0;
+    const bool dead = !inside_zs || (dk != 0 && !kp.periodic[0]) ||
+                      (dl != 0 && !kp.periodic[1]);
+    out = dead ? -1 : pack(k + dk, l + dl, m);
   } else if (k < 0 || l < 0 || m < 0 || k >= g.SX || l >= g.SY || m >= g.SZ) {
     out = -1;  // left the array: not representable
   }
