@@ -456,6 +456,18 @@ struct PassK {
 
 // the particle pass transposed for one species: lp (6 doubles per slot)
 // holds the adjoints after the step and returns those before it
+// B1 and B0 adjoints += the particles' gather-transpose scatter held in S
+__global__ void k_add_gather_T(int64_t n, Lam L) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double d = L.S[a][t];
+    L.B[a][t] += d;
+    L.B0[a][t] += d;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
                                                 Lam L, int sp, PassK K,
                                                 double *lp, double *gV_out) {
@@ -509,12 +521,10 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
         st_gather(g, F.f[c], s, gr);
         const double w = lf[c] * K.inv3;
         for (int a = 0; a < 3; ++a) lx[a] += w * gr[a];
-        if (c < 3) {
-          st_scatter(g, L.E[c], s, w);
-        } else {
-          st_scatter(g, L.B[c - 3], s, w);
-          st_scatter(g, L.B0[c - 3], s, w);
-        }
+        // the total B = B0 + B1 is gathered: both adjoints receive the same
+        // scatter, collected once in the scratch planes S and added to each
+        // after the pass (k_add_gather_T)
+        st_scatter(g, c < 3 ? L.E[c] : L.S[c - 3], s, w);
       }
     } else {
       for (int a = 0; a < 3; ++a) lv[a] = lvp[a];
@@ -1236,6 +1246,9 @@ extern "C" int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]) {
                                          tempB, dth, sc);
       k_halo_x_fwd<<<nb((int64_t)g.SY * g.SZ), kBlock, 0, st>>>(g, F.f[3], F.f[4], F.f[5]);
       k_halo_y_fwd<<<nb((int64_t)g.SX * g.SZ), kBlock, 0, st>>>(g, F.f[3], F.f[4], F.f[5]);
+      // S (free between the transposed E and B updates) collects the B part
+      // of the transposed gather
+      TRYH(hipMemsetAsync(L.S[0], 0, 3 * n * sizeof(double), st));
       for (int sp = 0; sp < 2; ++sp) {
         const int64_t np = ts[j].n[sp];
         if (np == 0) continue;
@@ -1256,6 +1269,7 @@ extern "C" int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]) {
         pk.bk.imf_y = c.imf[1];
         k_pass_T<<<nb(np), 256, 0, st>>>(g, tape_pd(j, sp), np, F, L, sp, pk, lp[sp], gvd);
       }
+      k_add_gather_T<<<nb(n), kBlock, 0, st>>>(n, L);
       TRYH(hipMemsetAsync(L.J[0][0], 0, 6 * n * sizeof(double), st));
       // sr_b and the first renew_b4_2
       k_halo_T_y<<<nb((int64_t)g.SX * g.SZ), kBlock, 0, st>>>(g, L.B[0], L.B[1], L.B[2],
