@@ -927,6 +927,127 @@ __global__ void k_refill32(Geom g, Particles32 P, int64_t base, int64_t count,
              draw_particle(seed, sp, event, (uint64_t)b), vth, mu0, mu1, mu2);
 }
 
+// In-tile inflow.  The slab cells the inflow empties (two z layers at each
+// open face) are refilled with as many particles per cell, and both lie in
+// the first and last z tiles.  Instead of appending the batch after the
+// sorted slots, each slab tile's share of it goes into that tile's dead
+// slots (killed or lost particles since the sort), so the next passes find
+// it inside the tile's window and no dead slots accumulate; what does not
+// fit is appended.  Particle j of the share of tile (tx, ty) at face f is
+// cell j / ppc of the tile's 2-layer slab (m fastest, then l, then k) and is
+// drawn from the counter of its reference-order index, as in k_refill32.
+struct SlabRefill {
+  Geom g;
+  TileGeom tg;
+  const uint32_t *tile_start;
+  int ppc, nface;
+  double vth, mu0, mu1, mu2;
+  uint64_t seed;
+  int sp;
+  uint32_t event;
+};
+
+__device__ inline int slab_tile(const SlabRefill &r, int blk, int &f, int &tx,
+                                int &ty) {
+  const int per_face = r.tg.NTX * r.tg.NTY;
+  f = blk / per_face;
+  const int t = blk - f * per_face;
+  tx = t / r.tg.NTY;
+  ty = t - tx * r.tg.NTY;
+  return (tx * r.tg.NTY + ty) * r.tg.NTZ + (f == 0 ? 0 : r.tg.NTZ - 1);
+}
+
+// rank of each dead slot of [s, e) among the dead slots, 256 at a time;
+// calls visit(slot, rank) for dead slots, returns the count
+template <typename Visit>
+__device__ inline int rank_dead(const int32_t *cell, int64_t s, int64_t e, Visit visit) {
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int running = 0;
+  for (int64_t i0 = s; i0 < e; i0 += kBlock) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool dead = i < e && cell[i] < 0;
+    const u64 bal = __builtin_amdgcn_ballot_w64(dead);
+    if (lane == 0) wsum[wv] = __builtin_popcountll(bal);
+    __syncthreads();
+    int base = running, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (w < wv) base += wsum[w];
+      total += wsum[w];
+    }
+    if (dead) visit(i, base + __builtin_popcountll(bal & ((1ull << lane) - 1ull)));
+    running += total;
+    __syncthreads();
+  }
+  return running;
+}
+
+__device__ inline void slab_particle(const SlabRefill &r, int f, int tx, int ty,
+                                     int j, Particles32 P, int64_t slot) {
+  const int c = j / r.ppc, i = j - c * r.ppc;
+  const int m = c & 1, rest = c >> 1;
+  const int l = ty * r.tg.TY + rest % r.tg.TY, k = tx * r.tg.TX + rest / r.tg.TY;
+  const int64_t per_face = (int64_t)r.g.NX * r.g.NY * 2 * r.ppc;
+  const int64_t b = f * per_face + (((int64_t)k * r.g.NY + l) * 2 + m) * r.ppc + i;
+  store_draw(P, slot, k + 2, l + 2, f == 0 ? m + 2 : m + r.g.NZ,
+             draw_particle(r.seed, r.sp, r.event, (uint64_t)b), r.vth, r.mu0, r.mu1,
+             r.mu2);
+}
+
+// pass 1: per slab tile, the particles that do not fit its dead slots
+__global__ __launch_bounds__(256) void k_slab_count32(SlabRefill r, const int32_t *cell,
+                                                      int *over) {
+  int f, tx, ty;
+  const int t = slab_tile(r, blockIdx.x, f, tx, ty);
+  const int share = r.tg.TX * r.tg.TY * 2 * r.ppc;
+  const int dead = rank_dead(cell, r.tile_start[t], r.tile_start[t + 1],
+                             [](int64_t, int) {});
+  if (threadIdx.x == 0) over[blockIdx.x] = share > dead ? share - dead : 0;
+}
+
+// exclusive scan of the overflow counts (one workgroup), total in over[nb]
+__global__ __launch_bounds__(256) void k_slab_scan32(int *over, int nb) {
+  __shared__ int part[kBlock];
+  const int per = (nb + kBlock - 1) / kBlock;
+  const int a = threadIdx.x * per, b = min(a + per, nb);
+  int sum = 0;
+  for (int i = a; i < b; ++i) sum += over[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < kBlock; ++w) {
+      const int v = part[w];
+      part[w] = run;
+      run += v;
+    }
+    over[nb] = run;
+  }
+  __syncthreads();
+  int run = part[threadIdx.x];
+  for (int i = a; i < b; ++i) {
+    const int v = over[i];
+    over[i] = run;
+    run += v;
+  }
+}
+
+// pass 2: the share of each slab tile into its dead slots, the rest after
+// the stored slots at tail + its offset
+__global__ __launch_bounds__(256) void k_slab_refill32(SlabRefill r, Particles32 P,
+                                                       const int *over, int64_t tail) {
+  int f, tx, ty;
+  const int t = slab_tile(r, blockIdx.x, f, tx, ty);
+  const int share = r.tg.TX * r.tg.TY * 2 * r.ppc;
+  const int placed =
+      rank_dead(P.cell, r.tile_start[t], r.tile_start[t + 1], [&](int64_t slot, int rank) {
+        if (rank < share) slab_particle(r, f, tx, ty, rank, P, slot);
+      });
+  const int64_t base = tail + over[blockIdx.x];
+  for (int j = min(placed, share) + (int)threadIdx.x; j < share; j += kBlock)
+    slab_particle(r, f, tx, ty, j, P, base + (j - min(placed, share)));
+}
+
 // inflow deletion (:6019-6021): z < 4dx or z > NZ dx, on the cell+offset
 // representation: m + oz < 4 <=> m <= 3; m + oz > NZ <=> m > NZ or
 // (m == NZ and oz > 0)
@@ -1468,6 +1589,42 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
     return set_error(ctx, PIC_E_CAPACITY,
                      "inflow: %lld + %lld particles exceed capacity %lld",
                      (long long)ctx->n[sp], (long long)add, (long long)ctx->cap);
+  const TileGeom tg = tile_geom(ctx);
+  const int nslab = nface * tg.NTX * tg.NTY;
+  // in-tile refill: tiles current, the slab layers inside the end z tiles,
+  // slots kept by the adjoint's forward (it needs the batch contiguous)
+  const bool in_tile = add > 0 && !ctx->adj_nosort && ctx->sorted_end[sp] > 0 &&
+                       ctx->sorted_end[sp] <= ctx->n[sp] && tg.ntiles() > 1 &&
+                       tg.TZ >= 2 && g.NZ % tg.TZ == 0 && g.NX % tg.TX == 0 &&
+                       g.NY % tg.TY == 0 && (int64_t)nslab + 1 <= ctx->cap;
+  if (in_tile) {
+    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+    SlabRefill r;
+    r.g = g;
+    r.tg = tg;
+    r.tile_start = ctx->tile_start[sp];
+    r.ppc = c.ppc;
+    r.nface = nface;
+    r.vth = c.vth[sp];
+    r.mu0 = c.vdrift[0];
+    r.mu1 = c.vdrift[1];
+    r.mu2 = c.vdrift[2];
+    r.seed = c.seed;
+    r.sp = sp;
+    r.event = event;
+    int *over = (int *)ctx->keys;  // free between sorts
+    k_slab_count32<<<(unsigned)nslab, kBlock, 0, ctx->stream>>>(r, P.cell, over);
+    k_slab_scan32<<<1, kBlock, 0, ctx->stream>>>(over, nslab);
+    PIC_LAUNCH_CHECK(ctx);
+    PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, over + nslab, sizeof(int),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t appended = *(int *)ctx->host_counter;
+    k_slab_refill32<<<(unsigned)nslab, kBlock, 0, ctx->stream>>>(r, P, over, ctx->n[sp]);
+    PIC_LAUNCH_CHECK(ctx);
+    ctx->n[sp] += appended;
+    return 0;
+  }
   if (add > 0) {
     Particles32 P = p32(ctx, sp, ctx->cur[sp]);
     k_refill32<<<(unsigned)div_up(add, kBlock), kBlock, 0, ctx->stream>>>(

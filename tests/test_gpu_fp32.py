@@ -321,3 +321,37 @@ def test_inflow_batch_matches_fp64_as_set(preset):
         assert np.array_equal(np.floor(a[:, :3]), np.floor(b[:, :3])), sp
         assert np.abs(a[:, :3] - b[:, :3]).max() < 1.2e-7 * 2
         assert np.abs(a[:, 3:] - b[:, 3:]).max() <= 6e-8 * np.abs(b[:, 3:]).max()
+
+
+@pytest.mark.parametrize("deposit", [ps.PIC_DEPOSIT_ATOMIC, ps.PIC_DEPOSIT_DETERMINISTIC])
+def test_inflow_into_dead_slots_matches_fp64_as_set(deposit):
+    """With current tiles the FP32 form writes an inflow batch into the dead
+    slots of the slab tiles it belongs to (appending only what does not
+    fit): after load, sort and an inflow the live particles are the FP64
+    form's (which appends the batch) as a set, the same in both deposit
+    modes, and a second identical run places them in the same slots."""
+    out = {}
+    raws = []
+    for prec, dep in ((ps.PIC_FP32, deposit), (ps.PIC_FP64, ps.PIC_DEPOSIT_ATOMIC),
+                      (ps.PIC_FP32, deposit)):
+        s = ps.Simulation(cfg(1, precision=prec, deposit=dep))
+        s.load_particles()
+        s.sort(0)
+        s.sort(1)
+        n_sorted = [s.particle_count(sp) for sp in (0, 1)]
+        s.inflow()
+        if prec == ps.PIC_FP32:
+            raws.append([s.download_raw32(sp) for sp in (0, 1)])
+        out.setdefault(prec, ([s.download_particles(sp) for sp in (0, 1)], n_sorted))
+        s.close()
+    for sp in (0, 1):
+        a, b = out[ps.PIC_FP32][0][sp], out[ps.PIC_FP64][0][sp]
+        assert a.shape == b.shape
+        # every cell holds the same number of particles
+        ca, cb = (np.unique(np.floor(x[:, :3]), axis=0, return_counts=True) for x in (a, b))
+        assert np.array_equal(ca[0], cb[0]) and np.array_equal(ca[1], cb[1]), sp
+        for j in range(6):
+            tol = 2.4e-7 if j < 3 else 6e-8 * np.abs(b[:, j]).max()
+            assert np.abs(np.sort(a[:, j]) - np.sort(b[:, j])).max() <= tol, (sp, j)
+        for x, y in zip(raws[0][sp], raws[1][sp]):
+            assert np.array_equal(x, y), sp
