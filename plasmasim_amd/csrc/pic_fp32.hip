@@ -1032,20 +1032,24 @@ __global__ __launch_bounds__(256) void k_slab_scan32(int *over, int nb) {
   }
 }
 
-// pass 2: the share of each slab tile into its dead slots, the rest after
-// the stored slots at tail + its offset
+// pass 2: the share of each slab tile into its dead slots (their indices
+// collected in LDS first, so that every lane draws), the rest after the
+// stored slots at tail + its offset; dynamic LDS of share ints
 __global__ __launch_bounds__(256) void k_slab_refill32(SlabRefill r, Particles32 P,
                                                        const int *over, int64_t tail) {
+  extern __shared__ int dead_slot[];
   int f, tx, ty;
   const int t = slab_tile(r, blockIdx.x, f, tx, ty);
   const int share = r.tg.TX * r.tg.TY * 2 * r.ppc;
-  const int placed =
+  const int ndead =
       rank_dead(P.cell, r.tile_start[t], r.tile_start[t + 1], [&](int64_t slot, int rank) {
-        if (rank < share) slab_particle(r, f, tx, ty, rank, P, slot);
+        if (rank < share) dead_slot[rank] = (int)slot;  // slots fit int32
       });
+  __syncthreads();
+  const int placed = min(ndead, share);
   const int64_t base = tail + over[blockIdx.x];
-  for (int j = min(placed, share) + (int)threadIdx.x; j < share; j += kBlock)
-    slab_particle(r, f, tx, ty, j, P, base + (j - min(placed, share)));
+  for (int j = threadIdx.x; j < share; j += kBlock)
+    slab_particle(r, f, tx, ty, j, P, j < placed ? (int64_t)dead_slot[j] : base + (j - placed));
 }
 
 // inflow deletion (:6019-6021): z < 4dx or z > NZ dx, on the cell+offset
@@ -1596,7 +1600,8 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   const bool in_tile = add > 0 && !ctx->adj_nosort && ctx->sorted_end[sp] > 0 &&
                        ctx->sorted_end[sp] <= ctx->n[sp] && tg.ntiles() > 1 &&
                        tg.TZ >= 2 && g.NZ % tg.TZ == 0 && g.NX % tg.TX == 0 &&
-                       g.NY % tg.TY == 0 && (int64_t)nslab + 1 <= ctx->cap;
+                       g.NY % tg.TY == 0 && (int64_t)nslab + 1 <= ctx->cap &&
+                       (int64_t)tg.TX * tg.TY * 2 * c.ppc * sizeof(int) <= 65536;
   if (in_tile) {
     Particles32 P = p32(ctx, sp, ctx->cur[sp]);
     SlabRefill r;
@@ -1620,7 +1625,8 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
                                 hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const int64_t appended = *(int *)ctx->host_counter;
-    k_slab_refill32<<<(unsigned)nslab, kBlock, 0, ctx->stream>>>(r, P, over, ctx->n[sp]);
+    const size_t lds = (size_t)tg.TX * tg.TY * 2 * c.ppc * sizeof(int);
+    k_slab_refill32<<<(unsigned)nslab, kBlock, lds, ctx->stream>>>(r, P, over, ctx->n[sp]);
     PIC_LAUNCH_CHECK(ctx);
     ctx->n[sp] += appended;
     return 0;
