@@ -45,7 +45,7 @@ BYTES_PER_PARTICLE_2D = {"fp64": 80, "fp32": 48}
 # FETCH_SIZE and WRITE_SIZE at each workload, scaled by the counts of a
 # 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md,
 # profiles/particle_pass_2d3v_2048sq_100ppc.md)
-TRAFFIC_PER_PARTICLE = {"fp64": 101.6, "fp32": 59.9}
+TRAFFIC_PER_PARTICLE = {"fp64": 101.6, "fp32": 58.0}
 TRAFFIC_PER_PARTICLE_2D = {"fp64": 81.4, "fp32": 49.0}
 
 
