@@ -20,8 +20,12 @@
  *   - Loading places exactly ppc particles per cell (the reference's list
  *     construction leaves one extra particle per thread list, :449-545).
  *
- * Unpinned parts (no reference test, fixture or closed form covers them):
- * the softened dipole core (r < dipole_soft), a config-level extension.
+ * Pinning: every routine is checked on the CPU against known answers and
+ * closed forms (tests/test_oracle_pinning.py), including the softened dipole
+ * core (r < dipole_soft, a config-level extension: the point-dipole field at
+ * radius dipole_soft on the same ray) and the diagnostics (exact divergence
+ * of cubic fields, uniform-field energies and Lorentz forces).  Unpinned:
+ * none beyond the random streams, which no reference test or fixture fixes.
  */
 #include "pic_oracle_internal.h"
 

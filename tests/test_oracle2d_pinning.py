@@ -7,8 +7,9 @@ the DST-I against scipy's, the Poisson solve against a dense solve of the
 5-point Dirichlet system, bilinear interpolation against hat functions on
 each staggered lattice, the non-relativistic Boris step against its closed
 form rotation, deposits against an independent scatter, the (2,4) leapfrog
-against its numerical dispersion relation and CFL limit, and the boundary,
-inflow, sort and probe rules against hand-built cases.
+against its numerical dispersion relation and CFL limit, the boundary,
+inflow, sort and probe rules against hand-built cases, and the diagnostics
+against closed forms.
 """
 import math
 
@@ -487,3 +488,89 @@ def test_whole_step_keeps_gauss_law_after_poisson():
         res[pe] = np.abs(gauss_residual(o, c)[1:, 1:-1]).max()
     assert res[1] < 1e-13
     assert res[0] > 1e-4
+
+
+# ---------------------------------------------------------------- diagnostics
+def diag_cfg():
+    # constants away from 1, so that a dropped or misplaced factor shows
+    c = cfg(10, 20)
+    c.dx, c.dt, c.c = 0.5, 0.05, 2.0
+    c.eps0, c.mu0, c.q = 2.5, 0.7, 1.3
+    c.mass[0], c.mass[1] = 30.0, 0.9
+    c.weight[0], c.weight[1] = 0.02, 0.03
+    c.imf[0], c.imf[1], c.imf[2] = 0.1, -0.2, 0.3
+    return c
+
+
+def test_diagnose_divergence_residuals_exact_for_quadratic_fields():
+    # the two-point differences are exact at the half node for quadratics:
+    # E_x = a x^2 gives 2 a (k + 1/2) dx; B1_x = b x^2 on its nodes gives
+    # 2 b (k - 1/2) dx (backward difference)
+    c = diag_cfg()
+    o = Oracle2D(c)
+    dx = c.dx
+    K, M = np.meshgrid(*[np.arange(s, dtype=float) for s in o.shape], indexing="ij")
+    a, g, b, h = 0.3, -0.7, 0.45, 0.2
+    imf = np.array(c.imf[:])
+    o.upload_field("ex", a * (K * dx) ** 2)
+    o.upload_field("ez", g * (M * dx) ** 2)
+    o.upload_field("bx", imf[0] + b * (K * dx) ** 2)
+    o.upload_field("bz", imf[2] + h * (M * dx) ** 2)
+    d = o.diagnose()
+    k = np.arange(2, c.nx + 2)[:, None]
+    m = np.arange(2, c.nz + 2)[None, :]
+    r = 2 * a * (k + 0.5) * dx + 2 * g * (m + 0.5) * dx
+    db = 2 * b * (k - 0.5) * dx + 2 * h * (m - 0.5) * dx
+    assert d["gauss_max"] == pytest.approx(np.abs(r).max(), rel=1e-12)
+    assert d["gauss_l2"] == pytest.approx(math.sqrt((r ** 2).sum()), rel=1e-12)
+    assert d["divb_max"] == pytest.approx(np.abs(db).max(), rel=1e-12)
+    assert d["divb_l2"] == pytest.approx(math.sqrt((db ** 2).sum()), rel=1e-12)
+
+
+def test_diagnose_gauss_residual_cancels_against_charge():
+    c = diag_cfg()
+    o = Oracle2D(c)
+    K = np.meshgrid(*[np.arange(s, dtype=float) for s in o.shape], indexing="ij")[0]
+    a = 0.9
+    o.upload_field("ex", a * K * c.dx)
+    o.upload_field("ne", np.full(o.shape, -c.eps0 * a / c.q))
+    ncell = c.nx * c.nz
+    d = o.diagnose()
+    assert d["gauss_l2"] < 1e-12 * a * math.sqrt(ncell)
+    assert d["charge"] == pytest.approx(c.eps0 * a * ncell * c.dx ** 2, rel=1e-12)
+
+
+def test_diagnose_uniform_field_energies_and_particle_sums():
+    c = diag_cfg()
+    o = Oracle2D(c)
+    imf = np.array(c.imf[:])
+    for i, n in enumerate(["b0x", "b0y", "b0z"]):
+        assert np.all(o.download_field(n) == imf[i])
+    E = np.array([0.4, -1.1, 0.25])
+    B = np.array([-0.3, 0.8, 1.7])
+    for i in range(3):
+        o.upload_field("ex ey ez".split()[i], np.full(o.shape, E[i]))
+        o.upload_field("bx by bz".split()[i], np.full(o.shape, B[i]))
+    rng = np.random.default_rng(9)
+    parts = {}
+    for sp in (0, 1):
+        n = 7 + 4 * sp
+        x = rng.uniform(2, c.nx + 2, n) * c.dx
+        z = rng.uniform(2, c.nz + 2, n) * c.dx
+        v = rng.standard_normal((n, 3)) * 0.3
+        parts[sp] = np.column_stack([x, z, v])
+        o.upload_particles(sp, parts[sp])
+    d = o.diagnose()
+    ncell, da = c.nx * c.nz, c.dx ** 2
+    assert d["field_energy_e"] == pytest.approx(0.5 * c.eps0 * (E @ E) * ncell * da, rel=1e-12)
+    assert d["field_energy_b"] == pytest.approx(0.5 * (B @ B) / c.mu0 * ncell * da, rel=1e-12)
+    b1 = B - imf
+    assert d["field_energy_b1"] == pytest.approx(0.5 * (b1 @ b1) / c.mu0 * ncell * da, rel=1e-12)
+    # uniform B: the difference terms cancel up to the rounding of their sum
+    assert d["divb_max"] < 1e-15 and d["charge"] == 0.0
+    for sp in (0, 1):
+        v = parts[sp][:, 2:]
+        mw = c.mass[sp] * c.weight[sp] * da
+        assert d["count"][sp] == len(v)
+        assert d["kinetic"][sp] == pytest.approx(0.5 * mw * np.einsum("ij,ij->", v, v), rel=1e-12)
+        assert np.allclose(d["momentum"][sp], mw * v.sum(0), rtol=1e-12, atol=0)

@@ -4,8 +4,9 @@ The reference ships no tests, fixtures or golden vectors, so each oracle
 function is checked against what the mathematics fixes: published Philox
 known-answer vectors, exactness of trilinear interpolation, the closed-form
 rotation of the reference's Boris variant, the (2,4) leapfrog dispersion
-relation, the sigma damping factors and brute-force deposits/sorts written
-independently of the oracle's index arithmetic.
+relation, the sigma damping factors, brute-force deposits/sorts written
+independently of the oracle's index arithmetic, the point dipole (its
+softened core included) and closed-form diagnostics.
 """
 import math
 
@@ -405,3 +406,167 @@ def test_b0_matches_point_dipole():
                 continue
             B = c.mu0 * M / (4 * math.pi) * (3 * (mhat @ r) * r - rr ** 2 * mhat) / rr ** 5
             assert abs(f[n][k, l, m] - B[i]) < 1e-12 * (1 + abs(B[i]))
+
+
+def test_b0_softened_core_is_the_field_at_the_core_radius():
+    # inside dipole_soft cells of the centre, B0 at a node is the point-dipole
+    # field at radius dipole_soft on the ray through that node: constant along
+    # each ray and continuous at the core's edge
+    c = ps.preset(1)
+    o = Oracle(c)
+    M, a, s = c.dipole_m, c.dipole_alpha, c.dipole_soft
+    mhat = np.array([0.0, math.sin(a), math.cos(a)])
+    cx = (c.nx + 4) // 2
+    comps = ["b0x", "b0y", "b0z"]
+    f = {n: o.download_field(n) for n in comps}
+    checked = 0
+    for k in range(cx - 3, cx + 4):
+        for l in range(cx - 3, cx + 4):
+            for m in range(cx - 3, cx + 4):
+                for i, n in enumerate(comps):
+                    off = np.zeros(3)
+                    off[i] = 0.5
+                    r = np.array([k, l, m]) + off - cx
+                    rr = np.linalg.norm(r)
+                    if rr >= s - 1e-9:
+                        continue
+                    u = r * (s / rr)
+                    B = c.mu0 * M / (4 * math.pi) * (3 * (mhat @ u) * u - s ** 2 * mhat) / s ** 5
+                    assert abs(f[n][k, l, m] - B[i]) < 1e-12 * (1 + abs(B[i])), (k, l, m, n)
+                    checked += 1
+    assert checked >= 30
+
+
+# ---------------------------------------------------------------- diagnostics
+def diag_cfg():
+    # constants away from 1, so that a dropped or misplaced factor shows
+    c = cfg_fields(10, 10, 20)
+    c.dx, c.dt, c.c = 0.5, 0.05, 2.0
+    c.eps0, c.mu0, c.q = 2.5, 0.7, 1.3
+    c.mass[0], c.mass[1] = 30.0, 0.9
+    c.weight[0], c.weight[1] = 0.02, 0.03
+    c.imf[0], c.imf[1], c.imf[2] = 0.1, -0.2, 0.3
+    return c
+
+
+def in_box(n):
+    # interior indices i with 0.2 n < i < 0.8 n (strict, as thrust_f)
+    return sum(1 for i in range(n) if 0.2 * n < i < 0.8 * n)
+
+
+def test_diagnose_gauss_residual_exact_for_cubic_e():
+    # the (2,4) divergence is exact for cubics: E_x = a x^3 on its nodes
+    # gives d/dx at the cell's half node, 3 a ((k + 1/2) dx)^2
+    c = diag_cfg()
+    o = Oracle(c)
+    dx = c.dx
+    K, L, M = np.meshgrid(*[np.arange(s, dtype=float) for s in o.shape], indexing="ij")
+    a, b, g = 0.3, -0.7, 0.11
+    o.upload_field("ex", a * (K * dx) ** 3)
+    o.upload_field("ey", b * (L * dx) ** 3)
+    o.upload_field("ez", g * (M * dx) ** 3)
+    d = o.diagnose()
+    k = np.arange(2, c.nx + 2)[:, None, None]
+    l = np.arange(2, c.ny + 2)[None, :, None]
+    m = np.arange(2, c.nz + 2)[None, None, :]
+    r = (3 * a * ((k + 0.5) * dx) ** 2 + 3 * b * ((l + 0.5) * dx) ** 2
+         + 3 * g * ((m + 0.5) * dx) ** 2)
+    assert d["gauss_max"] == pytest.approx(np.abs(r).max(), rel=1e-12)
+    assert d["gauss_l2"] == pytest.approx(math.sqrt((r ** 2).sum()), rel=1e-12)
+    assert d["charge"] == 0.0
+
+
+def test_diagnose_gauss_residual_cancels_against_charge():
+    # E_x = a x has div E = a; a uniform ion density eps0 a / q cancels it
+    c = diag_cfg()
+    o = Oracle(c)
+    K = np.meshgrid(*[np.arange(s, dtype=float) for s in o.shape], indexing="ij")[0]
+    a = 0.9
+    o.upload_field("ex", a * K * c.dx)
+    ncell = c.nx * c.ny * c.nz
+    d = o.diagnose()
+    assert d["gauss_max"] == pytest.approx(a, rel=1e-12)
+    assert d["gauss_l2"] == pytest.approx(a * math.sqrt(ncell), rel=1e-12)
+    o.upload_field("ni", np.full(o.shape, c.eps0 * a / c.q))
+    d = o.diagnose()
+    assert d["gauss_l2"] < 1e-12 * a * math.sqrt(ncell)
+    assert d["charge"] == pytest.approx(c.eps0 * a * ncell * c.dx ** 3, rel=1e-12)
+
+
+def test_diagnose_uniform_field_energies_and_box():
+    c = diag_cfg()
+    o = Oracle(c)
+    imf = np.array(c.imf[:])
+    for i, n in enumerate(["b0x", "b0y", "b0z"]):
+        assert np.all(o.download_field(n) == imf[i])
+    E = np.array([0.4, -1.1, 0.25])
+    B = np.array([-0.3, 0.8, 1.7])
+    for i in range(3):
+        o.upload_field("ex ey ez".split()[i], np.full(o.shape, E[i]))
+        o.upload_field("bx by bz".split()[i], np.full(o.shape, B[i]))
+    d = o.diagnose()
+    ncell, dv = c.nx * c.ny * c.nz, c.dx ** 3
+    assert d["field_energy_e"] == pytest.approx(0.5 * c.eps0 * (E @ E) * ncell * dv, rel=1e-12)
+    assert d["field_energy_b"] == pytest.approx(0.5 * (B @ B) / c.mu0 * ncell * dv, rel=1e-12)
+    b1 = B - imf
+    assert d["field_energy_b1"] == pytest.approx(0.5 * (b1 @ b1) / c.mu0 * ncell * dv, rel=1e-12)
+    # the central box holds 5 x 5 x 11 cells of the 10 x 10 x 20 interior;
+    # the reference sums its energy densities there without the cell volume
+    nbox = in_box(c.nx) * in_box(c.ny) * in_box(c.nz)
+    assert nbox == 5 * 5 * 11
+    assert d["energy_box"][0] == pytest.approx(0.5 * c.eps0 * (E @ E) * nbox, rel=1e-12)
+    assert d["energy_box"][1] == pytest.approx(0.5 * (B @ B) / c.mu0 * nbox, rel=1e-12)
+
+
+def test_diagnose_particle_sums_in_uniform_fields():
+    c = diag_cfg()
+    o = Oracle(c)
+    E = np.array([0.4, -1.1, 0.25])
+    B = np.array([-0.3, 0.8, 1.7])
+    for i in range(3):
+        o.upload_field("ex ey ez".split()[i], np.full(o.shape, E[i]))
+        o.upload_field("bx by bz".split()[i], np.full(o.shape, B[i]))
+    rng = np.random.default_rng(5)
+    dx, dv = c.dx, c.dx ** 3
+    L = np.array([c.nx, c.ny, c.nz]) * dx
+    parts = {}
+    for sp in (0, 1):
+        # half inside the 20-80 % box, half outside it along one axis (the
+        # box bounds are 0.2 and 0.8 of the interior length on the array's
+        # coordinates, whose interior starts at 2 dx, as thrust_f)
+        pos = rng.uniform(0.35, 0.65, (12, 3)) * L
+        for j in range(6):
+            pos[j, j % 3] = (0.9 * L[j % 3] + 2 * dx) if j % 2 else 0.1 * L[j % 3]
+        beta = rng.uniform(0.05, 0.8, 12)
+        u = rng.standard_normal((12, 3))
+        vel = (u / np.linalg.norm(u, axis=1)[:, None]) * (beta * c.c)[:, None]
+        parts[sp] = np.hstack([pos, vel])
+        o.upload_particles(sp, parts[sp])
+    d = o.diagnose()
+    F = np.zeros(15)
+    for sp in (0, 1):
+        p = parts[sp]
+        msp, w = c.mass[sp], c.weight[sp]
+        v = p[:, 3:]
+        b2 = (v ** 2).sum(1) / c.c ** 2
+        gam = 1 / np.sqrt(1 - b2)
+        # gamma - 1 in the cancellation-free form beta^2 gamma^2 / (gamma + 1)
+        assert d["kinetic"][sp] == pytest.approx(
+            (msp * c.c ** 2 * w * dv * b2 * gam ** 2 / (gam + 1)).sum(), rel=1e-12)
+        assert d["kinetic_nr"][sp] == pytest.approx(
+            (0.5 * msp * w * dv * (v ** 2).sum(1)).sum(), rel=1e-12)
+        assert np.allclose(d["momentum"][sp], (gam[:, None] * msp * w * dv * v).sum(0),
+                           rtol=1e-12, atol=0)
+        assert d["count"][sp] == len(p)
+        inside = np.all((p[:, :3] > 0.2 * L) & (p[:, :3] < 0.8 * L), axis=1)
+        assert inside.sum() == 6
+        qn = (1 if sp == 0 else -1) * c.q * w * dv
+        fe = qn * E * inside.sum()
+        fb = qn * np.cross(v[inside], B).sum(0)
+        b = 3 if sp == 0 else 9
+        F[b:b + 3] += fe
+        F[b + 3:b + 6] += fb
+        F[0:3] += fe + fb
+        assert d["energy_box"][2 + sp] == pytest.approx(
+            (0.5 * msp * w * dv * (v[inside] ** 2).sum(1)).sum(), rel=1e-12)
+    assert np.allclose(d["force"], F, rtol=1e-12, atol=1e-15)
