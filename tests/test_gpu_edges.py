@@ -123,3 +123,39 @@ def test_ragged_counts(precision, counts):
     for sp, n in enumerate(counts):
         assert g.particle_count(sp) == o.particle_count(sp)
         compare_particles(g, o, sp, precision)
+
+
+@pytest.mark.parametrize("n", [3, 4, 1027])
+def test_inflow_kill_with_ragged_counts(n):
+    """Inflow on uploaded particles spread over the whole flow axis, counts
+    that are not (or are just) a multiple of the kill kernel's four slots
+    per thread: the FP32 form against the oracle.  The survivors (between
+    the slabs) and the refilled slabs hold the same particles."""
+    c32 = cfg(ps.PIC_FP32)
+    g = ps.Simulation(c32)
+    o = Oracle(cfg(ps.PIC_FP64))
+    rng = np.random.default_rng(n)
+    for sp in (0, 1):
+        m = n + 5 * sp
+        p = np.column_stack([rng.uniform(2, c32.nx + 2, m) * c32.dx,
+                             rng.uniform(2, c32.ny + 2, m) * c32.dx,
+                             rng.uniform(2.01, c32.nz + 1.99, m) * c32.dx,
+                             rng.standard_normal((m, 3)) * 0.05])
+        # keep clear of the kill zone's edges (z = 4 dx and z = nz dx), where
+        # the fp32 offsets could round across
+        p = p[(np.abs(p[:, 2] - 4 * c32.dx) > 1e-3) & (np.abs(p[:, 2] - c32.nz * c32.dx) > 1e-3)]
+        for s in (g, o):
+            s.upload_particles(sp, p)
+    g.inflow()
+    o.inflow()
+    for sp in (0, 1):
+        assert g.particle_count(sp) == o.particle_count(sp)
+        a, b = g.download_particles(sp), o.download_particles(sp)
+        assert a.shape == b.shape
+        lo, hi = 4 * c32.dx, c32.nz * c32.dx
+        for part in (lambda q: (q[:, 2] >= lo) & (q[:, 2] <= hi),
+                     lambda q: (q[:, 2] < lo) | (q[:, 2] > hi)):
+            pa, pb = a[part(a)], b[part(b)]
+            assert pa.shape == pb.shape, sp
+            for j in range(6):
+                assert np.abs(np.sort(pa[:, j]) - np.sort(pb[:, j])).max(initial=0) < 1e-5, (sp, j)
