@@ -1091,18 +1091,45 @@ __global__ void k_live_flags32(const int32_t *__restrict__ cell, int64_t n,
   flags[i] = cell[i] >= 0 ? 1 : 0;
 }
 
+// dst[i] = src[idx[i]], four slots per thread: the destination and the
+// index are contiguous (16-byte loads and stores; the arrays start on
+// 16-byte boundaries), the sources are gathered
+template <typename T, typename V>
+__device__ inline void gather4(T *__restrict__ d, const T *__restrict__ s,
+                               const uint4 &j) {
+  V v;
+  v.x = s[j.x];
+  v.y = s[j.y];
+  v.z = s[j.z];
+  v.w = s[j.w];
+  *(V *)d = v;
+}
+
 __global__ void k_gather32(Particles32 dst, Particles32 src,
                            const uint32_t *__restrict__ idx, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t j = idx[i];
-  dst.cell[i] = src.cell[j];
-  dst.ox[i] = src.ox[j];
-  dst.oy[i] = src.oy[j];
-  dst.oz[i] = src.oz[j];
-  dst.vx[i] = src.vx[j];
-  dst.vy[i] = src.vy[j];
-  dst.vz[i] = src.vz[j];
+  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= n) return;
+  if (i0 + 4 <= n) {
+    const uint4 j = *(const uint4 *)(idx + i0);
+    gather4<int32_t, int4>(dst.cell + i0, src.cell, j);
+    gather4<float, float4>(dst.ox + i0, src.ox, j);
+    gather4<float, float4>(dst.oy + i0, src.oy, j);
+    gather4<float, float4>(dst.oz + i0, src.oz, j);
+    gather4<float, float4>(dst.vx + i0, src.vx, j);
+    gather4<float, float4>(dst.vy + i0, src.vy, j);
+    gather4<float, float4>(dst.vz + i0, src.vz, j);
+    return;
+  }
+  for (int64_t i = i0; i < n; ++i) {
+    const uint32_t j = idx[i];
+    dst.cell[i] = src.cell[j];
+    dst.ox[i] = src.ox[j];
+    dst.oy[i] = src.oy[j];
+    dst.oz[i] = src.oz[j];
+    dst.vx[i] = src.vx[j];
+    dst.vy[i] = src.vy[j];
+    dst.vz[i] = src.vz[j];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_sort_keys32(
@@ -1583,7 +1610,7 @@ static int compact(pic_ctx *ctx, int sp) {
   if (rc) return rc;
   Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
   if (kept > 0)
-    k_gather32<<<(unsigned)div_up(kept, kBlock), kBlock, 0, ctx->stream>>>(
+    k_gather32<<<(unsigned)div_up(kept, 4 * kBlock), kBlock, 0, ctx->stream>>>(
         dst, src, ctx->idx, kept);
   PIC_LAUNCH_CHECK(ctx);
   ctx->cur[sp] = 1 - ctx->cur[sp];
@@ -1693,7 +1720,7 @@ int sort(pic_ctx *ctx, int sp) {
   const int64_t valid = (int64_t)(*(uint32_t *)ctx->host_counter);
   Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
   if (valid > 0)
-    k_gather32<<<(unsigned)div_up(valid, kBlock), kBlock, 0, ctx->stream>>>(
+    k_gather32<<<(unsigned)div_up(valid, 4 * kBlock), kBlock, 0, ctx->stream>>>(
         dst, src, ctx->idx, valid);
   PIC_LAUNCH_CHECK(ctx);
   ctx->cur[sp] = 1 - ctx->cur[sp];
