@@ -456,7 +456,6 @@ __device__ __forceinline__ void particle_step(
   if (cl < 0) return;
   int k, l, m;
   unpack(cl, k, l, m);
-  const bool lds_ok = in_win_s(w, k, l, m);
   // fixed-point scale of the global path; in deterministic mode both paths
   // form each node's contribution with the same products in the same order,
   // so the sums do not depend on which particles the windows hold
@@ -482,6 +481,7 @@ __device__ __forceinline__ void particle_step(
     drift_axis(l, oy, vy, kp.hdt);
     drift_axis(m, oz, vz, kp.hdt);
     const bool dep = inside();
+    const bool lds_ok = in_win_s(w, k, l, m);
     if (dep && lds_ok)
       deposit_lds(w, wk, acc, true, k, l, m, ox, oy, oz, vx, vy, vz, kp);
     const float qs = kp.qw * fs;
@@ -493,6 +493,7 @@ __device__ __forceinline__ void particle_step(
     drift_axis(l, oy, vy, kp.hdt);
     drift_axis(m, oz, vz, kp.hdt);
     const bool dep = inside();
+    const bool lds_ok = in_win_s(w, k, l, m);
     if (dep && lds_ok)
       deposit_lds(w, wk, acc, false, k, l, m, ox, oy, oz, vx, vy, vz, kp);
     glob_coop(g, dep && !lds_ok, false, k, l, m, ox, oy, oz, kp.w * fs, 0.0f, 0.0f,
@@ -556,12 +557,16 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
   w.m0 = sw.m0;
   w.WY = __builtin_amdgcn_readfirstlane(sw.WY);
   w.WZ = __builtin_amdgcn_readfirstlane(sw.WZ);
-  w.vk0 = sw.vk0;
-  w.vk1 = sw.vk1;
-  w.vl0 = sw.vl0;
-  w.vl1 = sw.vl1;
-  w.vm0 = sw.vm0;
-  w.vm1 = sw.vm1;
+  // this pass tests each deposit's own cell (after its drift), not the
+  // cell at the start of the step: a deposit at cell c touches nodes c - 1
+  // to c + 1, so every cell within the anchor +- (kMargin - 1) keeps its
+  // nodes inside the window, one cell more than the start-of-step band
+  w.vk0 = sw.vk0 - 1;
+  w.vk1 = sw.vk1 + 1;
+  w.vl0 = sw.vl0 - 1;
+  w.vl1 = sw.vl1 + 1;
+  w.vm0 = sw.vm0 - 1;
+  w.vm1 = sw.vm1 + 1;
   const int WX = __builtin_amdgcn_readfirstlane(sw.WX);
   const int WN = w.on ? WX * w.WY * w.WZ : 0;
   for (int i = tid; i < 4 * WN; i += blockDim.x) acc_mem[(i / WN) * kWin + i % WN] = 0;
