@@ -126,6 +126,48 @@ def test_particle_routine_vs_fp64(state, routine, sp):
     assert rel_l2(a[:, 3:], b[:, 3:]) < 1e-6
 
 
+
+@pytest.mark.parametrize("sp", [0, 1])
+def test_ballistic_bounds_on_every_axis_vs_fp64(state, sp):
+    """The default workloads bound the ballistic zone (no gather, no push)
+    on the flow axis only; here its x and y bounds lie inside the array too,
+    so the pass tests all three axes.  One push against the FP64 form: J to
+    1e-5, particles as in the push test, and the particles inside the x/y
+    bounds' complement keep their velocities exactly."""
+    f, p = state
+    sims = []
+    for prec in (ps.PIC_FP32, ps.PIC_FP64):
+        c = cfg(1, precision=prec)
+        c.ballistic_lo[0], c.ballistic_hi[0] = 9.5 * c.dx, (c.nx - 5.5) * c.dx
+        c.ballistic_lo[1], c.ballistic_hi[1] = 7.25 * c.dx, (c.ny - 3.75) * c.dx
+        s = ps.Simulation(c)
+        for n, a in f.items():
+            s.upload_field(n, a)
+        for q in (0, 1):
+            s.upload_particles(q, p[q])
+        s.renew_grid()
+        s.push_deposit_j(sp)
+        sims.append(s)
+    s32, s64 = sims
+    for n in J:
+        a, b = s32.download_field(n), s64.download_field(n)
+        if np.linalg.norm(b) == 0:
+            continue
+        assert rel_l2(a, b) < 1e-5, (sp, n)
+    a, b = s32.download_particles(sp), s64.download_particles(sp)
+    assert a.shape == b.shape
+    assert np.abs(a[:, :3] - b[:, :3]).max() < 1e-5
+    assert rel_l2(a[:, 3:], b[:, 3:]) < 1e-6
+    c = cfg(1)
+    x, y = p[sp][:, 0], p[sp][:, 1]
+    out = ((x < 9.5 * c.dx) | (x > (c.nx - 5.5) * c.dx) | (y < 7.25 * c.dx) |
+           (y > (c.ny - 3.75) * c.dx))
+    assert 0.2 < out.mean() < 0.9
+    # the FP64 form keeps their velocities exactly; the FP32 form to fp32
+    # rounding of the uploaded values
+    assert np.array_equal(b[out, 3:], p[sp][out, 3:])
+    assert np.abs(a[out, 3:] - p[sp][out, 3:]).max() <= 1e-7 * np.abs(p[sp][:, 3:]).max()
+
 def developed_state(steps=20):
     """Oracle state after `steps` FP64 steps of the 32^3 workload."""
     c64 = cfg(1, precision=ps.PIC_FP64)
