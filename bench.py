@@ -45,11 +45,11 @@ BYTES_PER_PARTICLE_2D = {"fp64": 80, "fp32": 48}
 # FETCH_SIZE and WRITE_SIZE at each workload, scaled by the counts of a
 # 1 GiB copy kernel (profiles/particle_pass_3d_128cubed_64ppc.md,
 # profiles/particle_pass_2d3v_2048sq_100ppc.md)
-TRAFFIC_PER_PARTICLE = {"fp64": 101.6, "fp32": 58.0}
+TRAFFIC_PER_PARTICLE = {"fp64": 101.7, "fp32": 58.2}
 TRAFFIC_PER_PARTICLE_2D = {"fp64": 81.4, "fp32": 49.0}
 
 
-def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0):
+def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None):
     if workload == "2d3v":
         c = ps.preset2d(3)
         c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
@@ -58,6 +58,8 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0):
         c = ps.preset(2)
         c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
         c.poisson_every = poisson_every
+        if tile:
+            c.tile[0], c.tile[1], c.tile[2] = tile
         sim = ps.Simulation(c)
     sim.load_particles()
     sim.begin()
@@ -293,6 +295,8 @@ def main():
     ap.add_argument("--poisson-every", type=int, default=0, metavar="P",
                     help="3d: run the 3D Poisson correction every P steps (0: off, "
                          "as the reference)")
+    ap.add_argument("--tile", type=int, nargs=3, metavar=("TX", "TY", "TZ"),
+                    help="sort tile of the 3D workload in cells (default: the preset's, 8 4 4)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -332,10 +336,10 @@ def main():
                                    "x 2 species, file under $TMPDIR"},
             "checkpoint": r}))
         return
-    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload, a.poisson_every)
+    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload, a.poisson_every, a.tile)
     other = None
     if a.form == "fp32" and not a.no_fp64:
-        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload, a.poisson_every)
+        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload, a.poisson_every, a.tile)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(ps) if a.workload == "3d" else cpu_baseline_2d(ps)
@@ -349,6 +353,8 @@ def main():
                     "sort every 20, inflow every 5")
         if a.poisson_every > 0:
             workload += f", Poisson correction every {a.poisson_every}"
+        if a.tile:
+            workload += ", sort tiles %d x %d x %d" % tuple(a.tile)
     out = {
         "metric": "particle-steps/s on 1 MI355X, FP32 vs FP64 form at stated field error",
         "value": main_r["rate"],
