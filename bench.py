@@ -4,8 +4,18 @@ Default workload (3d): 128^3 cells x 64 particles/cell/species (268,435,456
 particles), two species, dipole standoff 32 cells, inflow every 5 steps,
 cell sort every 20 steps, synthetic seeded plasma.  A step is one full
 fdtd() step (B half steps, fused gather/push/deposit pass per species,
-current fold, E step, particle boundary, and the amortised inflow and sort
-when due), with all state resident in HBM.
+current fold, E step, particle boundary, the amortised inflow and sort
+when due, and thrust_fave's per-step FP64 force sum over the central box),
+with all state resident in HBM.  The line also carries the field error of
+the FP32 form: the relative L2 of J, n, E and B - B0 against the FP64 form
+after one step from the same load.
+
+--workload config5: 256^3 cells x 32 ppc x 2 species (1,073,741,824
+particles, m_i/m_e = 400, drift 0.05, dipole tilted 30 degrees), FP32 form
+in the deterministic sorted-tile deposit mode and in the atomic mode:
+particle-steps/s and the particle pass's roofline fraction of each, and
+whether two deterministic runs of 21 steps (a sort and inflows included)
+end in bit-identical fields and diagnostics.
 
 --workload adjoint: gradient of the step-averaged Maxwell-stress flux
 through a sphere of 10 cells around the dipole with respect to the dipole
@@ -49,20 +59,26 @@ TRAFFIC_PER_PARTICLE = {"fp64": 101.7, "fp32": 58.2}
 TRAFFIC_PER_PARTICLE_2D = {"fp64": 81.4, "fp32": 49.0}
 
 
-def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None):
+def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None,
+             thrust=True, preset=2, deposit=None):
     if workload == "2d3v":
         c = ps.preset2d(3)
         c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
         sim = ps.Simulation2D(c)
     else:
-        c = ps.preset(2)
+        c = ps.preset(preset)
         c.precision = ps.PIC_FP64 if form == "fp64" else ps.PIC_FP32
         c.poisson_every = poisson_every
+        if deposit is not None:
+            c.deposit = deposit
         if tile:
             c.tile[0], c.tile[1], c.tile[2] = tile
         sim = ps.Simulation(c)
     sim.load_particles()
     sim.begin()
+    if workload != "2d3v" and thrust:
+        # thrust_fave every step (:926), as the reference's step loop
+        sim.thrust_ave(True)
     # live per-kernel timing over every step of the run (HIP events on the
     # library's stream, a few microseconds per step), so the kernel average
     # covers the same launches a kernel trace of this command does
@@ -82,6 +98,10 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None)
     push_ms, push_n = sim.timing_get("push")
     tot = {k: sim.timing_get(k) for k in ("push", "field_b", "field_e", "sort", "inflow", "other")}
     sim.timing(False)
+    thrust_sum = None
+    if workload != "2d3v" and thrust:
+        F, nsteps = sim.thrust_ave_get()
+        thrust_sum = {"steps": nsteps, "F0_2": F[:3].tolist()}
     sim.close()
     per_launch_particles = particles / 2.0  # one launch per species
     push_avg_s = push_ms / push_n / 1e3
@@ -94,7 +114,40 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None)
         "traffic": (TRAFFIC_PER_PARTICLE_2D if workload == "2d3v"
                     else TRAFFIC_PER_PARTICLE)[form] * per_launch_particles,
         "bytes_per_particle": bpp,
+        "thrust": thrust_sum,
     }
+
+
+def field_error(ps, preset=2, tile=None):
+    """Relative L2 of J, n, E and the induced field B - B0 of the FP32 form
+    against the FP64 form after one step from the same seeded load."""
+    import numpy as np
+
+    def rel_l2(x, y):
+        ny = np.linalg.norm(y)
+        return float(np.linalg.norm(x - y) / ny) if ny else float(np.linalg.norm(x - y))
+
+    sims = {}
+    for form, prec in (("fp32", ps.PIC_FP32), ("fp64", ps.PIC_FP64)):
+        c = ps.preset(preset)
+        c.precision = prec
+        if tile:
+            c.tile[0], c.tile[1], c.tile[2] = tile
+        s = ps.Simulation(c)
+        s.load_particles()
+        s.begin()
+        s.step(1)
+        sims[form] = s
+    a, b = sims["fp32"], sims["fp64"]
+    out = {}
+    for n in ("jix", "jiy", "jiz", "jex", "jey", "jez", "ni", "ne", "ex", "ey", "ez"):
+        out[n] = rel_l2(a.download_field(n), b.download_field(n))
+    for n in ("bx", "by", "bz"):
+        b0 = b.download_field("b0" + n[1])
+        out[n + "-b0"] = rel_l2(a.download_field(n) - b0, b.download_field(n) - b0)
+    a.close()
+    b.close()
+    return {"after_steps": 1, "rel_l2": out, "max": max(out.values())}
 
 
 def cpu_baseline(ps):
@@ -139,6 +192,45 @@ def calibrate_copy(mib):
     print(f"copy calibration: {mib} MiB read, {mib} MiB written", file=sys.stderr)
     del src, dst
     torch.cuda.empty_cache()
+
+
+def copy_bandwidth(nbytes=4 << 30, reps=5):
+    """Bytes per second of a plain device-to-device copy (the HIP runtime's
+    copy kernel, the runtime the library links), read + written, timed with
+    HIP events: the achievable HBM rate beside the 8 TB/s vendor peak."""
+    import ctypes
+    h = ctypes.CDLL("libamdhip64.so.7")
+    vp = ctypes.c_void_p
+    h.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    h.hipFree.argtypes = [vp]
+    h.hipMemset.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
+    h.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+    h.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    h.hipEventRecord.argtypes = [vp, vp]
+    h.hipEventSynchronize.argtypes = [vp]
+    h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    h.hipEventDestroy.argtypes = [vp]
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    if h.hipMalloc(ctypes.byref(a), nbytes) or h.hipMalloc(ctypes.byref(b), nbytes):
+        return None
+    try:
+        h.hipMemset(a, 1, nbytes)
+        h.hipEventCreate(ctypes.byref(e0))
+        h.hipEventCreate(ctypes.byref(e1))
+        h.hipMemcpyAsync(b, a, nbytes, 3, None)  # warm-up
+        h.hipEventRecord(e0, None)
+        for _ in range(reps):
+            h.hipMemcpyAsync(b, a, nbytes, 3, None)
+        h.hipEventRecord(e1, None)
+        h.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        h.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        h.hipEventDestroy(e0)
+        h.hipEventDestroy(e1)
+        return 2.0 * nbytes * reps / (ms.value * 1e-3)
+    finally:
+        h.hipFree(a)
+        h.hipFree(b)
 
 
 def cpu_baseline_2d(ps):
@@ -208,11 +300,19 @@ def run_adjoint(ps, steps=200, ckpt=20, radius=10.0, nquad=1024,
     g32, g64 = np.array(out["fp32"]["grad"]), np.array(out["fp64"]["grad"])
     out["fp32_vs_fp64_grad_rel"] = (np.abs(g32 - g64) / np.maximum(np.abs(g64), 1e-300)).tolist()
     sim = ps.Simulation(c)
+    c32, _ = make("fp32")
+    sim32 = ps.Simulation(c32)
     table = []
     for N in horizons:
         loss = sim.adj_forward(p, N, ckpt, radius, nquad)
         g = sim.adj_backward(1.0)
         row = {"steps": N, "loss": loss, "adjoint": g.tolist()}
+        # the FP32-forward gradient at the same horizon, against the FP64 one
+        sim32.adj_forward(p, N, ckpt, radius, nquad)
+        g32 = sim32.adj_backward(1.0)
+        row["adjoint_fp32_forward"] = g32.tolist()
+        row["fp32_vs_fp64_rel"] = [abs(a - b) / max(abs(b), 1e-300)
+                                   for a, b in zip(g32.tolist(), g.tolist())]
         for h_rel in (1e-4, 1e-7):
             fd = []
             for i in range(4):
@@ -226,10 +326,51 @@ def run_adjoint(ps, steps=200, ckpt=20, radius=10.0, nquad=1024,
             row[f"rel_err_h{h_rel:g}"] = [abs(a - b) / max(abs(a), 1e-300)
                                           for a, b in zip(fd, g.tolist())]
         table.append(row)
-        print(f"adjoint horizon {N}: rel err (h=1e-7) {row['rel_err_h1e-07']}",
-              file=sys.stderr, flush=True)
+        print(f"adjoint horizon {N}: rel err (h=1e-7) {row['rel_err_h1e-07']}, "
+              f"(h=1e-4) {row['rel_err_h0.0001']}, FP32 forward vs FP64 "
+              f"{row['fp32_vs_fp64_rel']}", file=sys.stderr, flush=True)
     sim.close()
+    sim32.close()
     out["horizons"] = table
+    return out
+
+
+def run_config5(ps, steps, warmup):
+    """256^3 x 32 ppc FP32 in the deterministic and the atomic deposit mode
+    (rate and particle-pass roofline of each), and two deterministic runs of
+    21 steps compared bit for bit (every field plane, the diagnostics and
+    the counts)."""
+    import numpy as np
+    out = {}
+    for mode, dep in (("deterministic", ps.PIC_DEPOSIT_DETERMINISTIC),
+                      ("atomic", ps.PIC_DEPOSIT_ATOMIC)):
+        r = run_form(ps, "fp32", steps, warmup, preset=5, deposit=dep)
+        out[mode] = {"value": r["rate"], "ms_per_step": r["ms_per_step"],
+                     "pass_avg_ms": r["push_avg_ms"],
+                     "frac": r["achieved"] / HBM_PEAK,
+                     "achieved_GBps": r["achieved"] / 1e9,
+                     "kernel_ms_per_step": r["kernel_ms"], "particles": r["particles"]}
+        print(f"config5 {mode}: {r['rate']:.3e} particle-steps/s, pass "
+              f"{r['push_avg_ms']:.2f} ms", file=sys.stderr, flush=True)
+    runs = []
+    for _ in range(2):
+        c = ps.preset(5)
+        c.deposit = ps.PIC_DEPOSIT_DETERMINISTIC
+        s = ps.Simulation(c)
+        s.load_particles()
+        s.begin()
+        s.step(21)  # inflows at 5, 10, 15, 20 and the sort at step 20
+        f = {n: s.download_field(n) for n in
+             ("ex", "ey", "ez", "bx", "by", "bz", "jix", "jiy", "jiz", "jex", "jey",
+              "jez", "ni", "ne")}
+        d = s.diagnose()
+        s.close()
+        runs.append((f, d))
+    (fa, da), (fb, db) = runs
+    same_fields = all(np.array_equal(fa[n], fb[n]) for n in fa)
+    same_diag = all(np.array_equal(np.asarray(da[k]), np.asarray(db[k])) for k in da)
+    out["deterministic_bit_identical"] = {"steps": 21, "fields": bool(same_fields),
+                                          "diagnostics_and_counts": bool(same_diag)}
     return out
 
 
@@ -290,8 +431,10 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--form", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--workload", choices=["3d", "2d3v", "adjoint", "checkpoint"],
+    ap.add_argument("--workload", choices=["3d", "2d3v", "adjoint", "checkpoint", "config5"],
                     default="3d")
+    ap.add_argument("--no-thrust", action="store_true",
+                    help="3d: leave out thrust_fave's per-step force sum")
     ap.add_argument("--poisson-every", type=int, default=0, metavar="P",
                     help="3d: run the 3D Poisson correction every P steps (0: off, "
                          "as the reference)")
@@ -310,6 +453,8 @@ def main():
     if a.calibrate_copy > 0:
         calibrate_copy(a.calibrate_copy)
     import plasmasim_amd as ps
+    ps.lib()
+    copy_Bps = copy_bandwidth()
     if a.workload == "adjoint":
         r = run_adjoint(ps)
         f32 = r["fp32"]
@@ -324,6 +469,25 @@ def main():
                                    "1024 quadrature points"},
             "adjoint": r}))
         return
+    if a.workload == "config5":
+        r = run_config5(ps, a.steps, a.warmup)
+        det = r["deterministic"]
+        print(json.dumps({
+            "metric": "particle-steps/s on 1 MI355X, FP32 deterministic sorted-tile deposit",
+            "value": det["value"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": det["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic seeded plasma (Philox4x32-10)",
+            "config": {"workload": "3D EM-PIC step, 256^3 cells x 32 ppc x 2 species "
+                                   "(1,073,741,824 particles), m_i/m_e 400, drift 0.05, "
+                                   "dipole tilted 30 degrees, sort every 20, inflow every 5"},
+            "roofline": {"bound": "hbm", "achieved": det["achieved_GBps"],
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": det["frac"],
+                         "traffic": None,
+                         "kernel": "fused particle pass, deterministic mode",
+                         "kernel_avg_ms": det["pass_avg_ms"], "bytes_per_particle": 56},
+            "config5": r}))
+        return
     if a.workload == "checkpoint":
         r = run_checkpoint(ps)
         print(json.dumps({
@@ -336,10 +500,16 @@ def main():
                                    "x 2 species, file under $TMPDIR"},
             "checkpoint": r}))
         return
-    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload, a.poisson_every, a.tile)
+    thrust = not a.no_thrust
+    main_r = run_form(ps, a.form, a.steps, a.warmup, a.workload, a.poisson_every, a.tile,
+                      thrust)
     other = None
     if a.form == "fp32" and not a.no_fp64:
-        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload, a.poisson_every, a.tile)
+        other = run_form(ps, "fp64", a.steps, a.warmup, a.workload, a.poisson_every, a.tile,
+                         thrust)
+    ferr = None
+    if a.workload == "3d" and a.form == "fp32" and not a.no_fp64:
+        ferr = field_error(ps, tile=a.tile)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(ps) if a.workload == "3d" else cpu_baseline_2d(ps)
@@ -351,6 +521,8 @@ def main():
         workload = ("3D EM-PIC step, 128^3 cells x 64 ppc x 2 species "
                     "(268,435,456 particles), dipole standoff 32, dt 0.45, "
                     "sort every 20, inflow every 5")
+        workload += (", thrust_fave force sum every step" if thrust
+                     else ", without thrust_fave")
         if a.poisson_every > 0:
             workload += f", Poisson correction every {a.poisson_every}"
         if a.tile:
@@ -373,12 +545,21 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": main_r["achieved"] / HBM_PEAK,
                      "traffic": main_r["traffic"],
+                     "traffic_source": "profile constant: FETCH_SIZE + WRITE_SIZE bytes "
+                                       "per particle from a rocprofv3 --pmc run of this "
+                                       "workload, times the particles of one launch",
                      "kernel": "fused particle pass (gather, push, J and n deposit, boundary)",
                      "kernel_avg_ms": main_r["push_avg_ms"],
-                     "bytes_per_particle": main_r["bytes_per_particle"]},
+                     "bytes_per_particle": main_r["bytes_per_particle"],
+                     "copy_measured": copy_Bps / 1e9 if copy_Bps else None,
+                     "frac_of_copy": main_r["achieved"] / copy_Bps if copy_Bps else None},
         "cpu_baseline": cpu,
         "kernel_ms_per_step": main_r["kernel_ms"],
     }
+    if ferr:
+        out["field_error_vs_fp64"] = ferr
+    if main_r.get("thrust"):
+        out["thrust_fave"] = main_r["thrust"]
     if other:
         out["fp64_value"] = other["rate"]
         out["fp64_ms_per_step"] = other["ms_per_step"]

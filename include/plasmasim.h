@@ -222,6 +222,20 @@ int pic_diagnose(pic_ctx *ctx, pic_diag *out);
  * and inflow cadences, fusing loops A and B and the particle boundary into
  * one particle pass per species. */
 int pic_step(pic_ctx *ctx, int nsteps);
+/* thrust_fave (:926, :5124-5285), every step of pic_step: the Lorentz force
+ * q (E + v x B) n dx^3 on the particles of the central box (0.2 N dx <
+ * x, y, z < 0.8 N dx) at the end of each step, summed over the steps since
+ * the last reset.  The force of step c is added during the particle pass
+ * of step c + 1 (same positions, velocities and E; the end-of-step B is
+ * kept before the first B half-step), so the summation costs no extra
+ * particle stream.  pic_thrust_ave_enable(on) resets the sum.
+ * pic_thrust_ave_get fills the reference's F[27] (F[0..2] total, F[3..8]
+ * ion E and v x B parts, F[9..14] electron parts, F[15..26] zero: every
+ * particle has flag 1) with the sum over *nsteps steps (the reference
+ * writes Fave = sum / 500 every 500 steps) and syncs; reset zeroes the sum.
+ * Routines called one by one add nothing: the sum covers pic_step only. */
+int pic_thrust_ave_enable(pic_ctx *ctx, int on);
+int pic_thrust_ave_get(pic_ctx *ctx, double F[27], int64_t *nsteps, int reset);
 /* The reference's pre-loop inflow (:673-674) followed by nothing else;
  * call once after pic_load_particles and before the first pic_step. */
 int pic_begin(pic_ctx *ctx);
@@ -231,6 +245,12 @@ int pic_begin(pic_ctx *ctx);
  * 7-point Laplacian symbol, inverse transforms; E += (2,4) gradient of phi).
  * PIC_E_STATE when poisson_every is 0 (no buffers). */
 int pic_poisson(pic_ctx *ctx);
+
+/* Species sorts done so far by each method: the FP32 form sorts with the
+ * tile sort (particles still tile-sorted since the last sort, each tile's
+ * movers within its list) and otherwise, as the FP64 form always does, with
+ * a radix sort of (key, slot) pairs.  Both give the stable key order. */
+int pic_sort_stats(const pic_ctx *ctx, int64_t *tile, int64_t *radix);
 
 /* Step counter (c) and cadence counters (c_add, c_thrust).  pic_step keeps
  * them; a host loop that calls the routines itself keeps its own (fdtd's
@@ -341,7 +361,10 @@ typedef struct pic_checkpoint_header {
   int64_t sorted_end[2];   /* tile-sorted prefix per species */
   uint64_t payload_bytes;
   uint64_t checksum;
-  uint8_t reserved[136];
+  int32_t drift_halves[2]; /* FP32 form: half-step drifts since the tile sort
+                              per species (the tile windows' drift shift);
+                              0 from writers without tile windows */
+  uint8_t reserved[128];
 } pic_checkpoint_header;
 
 int pic_checkpoint_save(pic_ctx *ctx, const char *path);

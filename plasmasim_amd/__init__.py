@@ -175,6 +175,24 @@ class Simulation:
         self._check(self._L.pic_diagnose(self._ctx, ctypes.byref(d)))
         return d.to_dict()
 
+    def sort_stats(self):
+        """(sorts by the tile sort, sorts by the radix sort) so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.pic_sort_stats(self._ctx, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def thrust_ave(self, on=True):
+        """pic_thrust_ave_enable: sum thrust_fave's force over pic_step's steps."""
+        self._check(self._L.pic_thrust_ave_enable(self._ctx, 1 if on else 0))
+
+    def thrust_ave_get(self, reset=False):
+        """(F[27] summed over the steps, number of steps)."""
+        F = np.zeros(27)
+        n = ctypes.c_int64()
+        self._check(self._L.pic_thrust_ave_get(self._ctx, _lib.dptr(F), ctypes.byref(n),
+                                               1 if reset else 0))
+        return F, n.value
+
     def sync(self):
         self._check(self._L.pic_sync(self._ctx))
 

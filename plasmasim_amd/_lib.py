@@ -163,7 +163,10 @@ SIGNATURES = [
     ("pic_sort", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_diagnose", ctypes.c_int, [_P, ctypes.POINTER(PicDiag)]),
     ("pic_step", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic_thrust_ave_enable", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic_thrust_ave_get", ctypes.c_int, [_P, _D, _I64P, ctypes.c_int]),
     ("pic_begin", ctypes.c_int, [_P]),
+    ("pic_sort_stats", ctypes.c_int, [_P, _I64P, _I64P]),
     ("pic_get_counters", ctypes.c_int,
      [_P, _I64P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     ("pic_set_counters", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),

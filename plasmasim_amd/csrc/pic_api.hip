@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include "pic_internal.h"
+#include "pic_tiles.h"
 
 using namespace pic;
 
@@ -319,6 +320,9 @@ static void free_all(pic_ctx *ctx) {
     if (ctx->tile_start[sp]) (void)hipFree(ctx->tile_start[sp]);
   if (ctx->host_counter) (void)hipHostFree(ctx->host_counter);
   if (ctx->diag_partial) (void)hipFree(ctx->diag_partial);
+  if (ctx->thrust_acc) (void)hipFree(ctx->thrust_acc);
+  if (ctx->thrust_part) (void)hipFree(ctx->thrust_part);
+  if (ctx->thrust_bprev) (void)hipFree(ctx->thrust_bprev);
   for (int i = 0; i < pic_ctx::kEventPool; ++i)
     for (int j = 0; j < 2; ++j)
       if (ctx->tev[j][i]) (void)hipEventDestroy(ctx->tev[j][i]);
@@ -784,6 +788,13 @@ extern "C" int pic_diagnose(pic_ctx *ctx, pic_diag *out) {
 
 extern "C" int pic_begin(pic_ctx *ctx) { return pic_inflow(ctx); }
 
+extern "C" int pic_sort_stats(const pic_ctx *ctx, int64_t *tile, int64_t *radix) {
+  if (!ctx) return PIC_E_ARG;
+  if (tile) *tile = ctx->sort_tile_count;
+  if (radix) *radix = ctx->sort_radix_count;
+  return 0;
+}
+
 extern "C" int pic_get_counters(const pic_ctx *ctx, int64_t *step,
                                 int32_t *c_add, int32_t *c_sort) {
   if (!ctx) return PIC_E_ARG;
@@ -799,6 +810,158 @@ extern "C" int pic_set_counters(pic_ctx *ctx, int64_t step, int32_t c_add,
   ctx->step = step;
   ctx->c_add = c_add;
   ctx->c_sort = c_sort;
+  return 0;
+}
+
+namespace pic {
+
+// ---------------------------------------------------------------------------
+// thrust_fave (:5124-5285) accumulated over pic_step's steps
+
+// acc[q] += qn * (sum over the nb workgroups' partials of quantity q), the
+// workgroups summed in a fixed order
+constexpr int kFinish = 1024;
+__global__ __launch_bounds__(kFinish) void k_thrust_finish(const double *__restrict__ part,
+                                                           int nb, double qn, double *acc) {
+  __shared__ double sh[6][kFinish];
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += kFinish)
+    for (int q = 0; q < 6; ++q) v[q] += part[(int64_t)b * 6 + q];
+  for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = kFinish / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) acc[threadIdx.x] += qn * sh[threadIdx.x][0];
+}
+
+// the force parts of the diagnostics partials (particle quantities 8..13,
+// already multiplied by q_s w dx^3) of both species added to acc
+__global__ __launch_bounds__(64) void k_thrust_from_diag(const double *__restrict__ pf,
+                                                         double *acc) {
+  const int t = threadIdx.x;
+  if (t >= 12) return;
+  const int sp = t / 6, q = t % 6;
+  const double *pp =
+      pf + (int64_t)kDiagNBF * kDiagNQF + (int64_t)sp * kDiagNBP * kDiagNQP;
+  double v = 0.0;
+  for (int b = 0; b < kDiagNBP; ++b) v += pp[(int64_t)b * kDiagNQP + 8 + q];
+  acc[t] += v;
+}
+
+__global__ void k_fill_zero(double *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+int thrust_add(pic_ctx *ctx, int sp, int nb, double qn) {
+  k_thrust_finish<<<1, kFinish, 0, ctx->stream>>>(ctx->thrust_part, nb, qn,
+                                                  ctx->thrust_acc + 6 * sp);
+  PIC_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
+// the pending force from the current state, by the diagnostics' own particle
+// gather (the state is still that of the end of the last step)
+static int thrust_settle(pic_ctx *ctx) {
+  if (!ctx->thrust_pending) return 0;
+  int rc = is_f64(ctx) ? f64::diagnose_partials(ctx, ctx->diag_partial, nullptr)
+                       : f32::diagnose_partials(ctx, ctx->diag_partial, nullptr);
+  if (rc) return rc;
+  k_thrust_from_diag<<<1, 64, 0, ctx->stream>>>(ctx->diag_partial, ctx->thrust_acc);
+  PIC_LAUNCH_CHECK(ctx);
+  ctx->thrust_pending = 0;
+  ctx->thrust_steps++;
+  return 0;
+}
+
+// keep the end-of-step total B for the fused force of the next pass: the
+// FP64 B planes into thrust_bprev, the FP32 form's total-B planes into the
+// upper half of their own 6-plane block
+static int thrust_keep_b(pic_ctx *ctx) {
+  const int64_t pl = ctx->geom.plane;
+  if (is_f64(ctx)) {
+    PIC_HIP(ctx, hipMemcpyAsync(ctx->thrust_bprev, fields<double>(ctx).f[PIC_BX],
+                                3 * pl * sizeof(double), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  } else {
+    float *bt = f32::total_b(ctx);
+    PIC_HIP(ctx, hipMemcpyAsync(bt + 3 * pl, bt, 3 * pl * sizeof(float),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return 0;
+}
+
+}  // namespace pic
+
+// No reference counterpart as an entry point: switches on thrust_fave's
+// per-step accumulation (:926, :5124-5285) inside pic_step.
+extern "C" int pic_thrust_ave_enable(pic_ctx *ctx, int on) {
+  if (!ctx) return PIC_E_ARG;
+  ctx->thrust_pending = 0;
+  ctx->thrust_now = 0;
+  ctx->thrust_steps = 0;
+  ctx->thrust_on = 0;
+  if (!on) return 0;
+  const Geom &g = ctx->geom;
+  const size_t esz = is_f64(ctx) ? sizeof(double) : sizeof(float);
+  const TileGeom tg = tile_geom(ctx);
+  // workgroups of one pass: the tiles and the tail chunks of up to the
+  // capacity's slots
+  const int64_t slots = (int64_t)tg.ntiles() + div_up(ctx->cap, kTailChunk) + 1;
+  if (!ctx->thrust_acc) PIC_HIP(ctx, hipMalloc(&ctx->thrust_acc, 16 * sizeof(double)));
+  if (ctx->thrust_part_slots < slots) {
+    if (ctx->thrust_part) (void)hipFree(ctx->thrust_part);
+    ctx->thrust_part = nullptr;
+    ctx->thrust_part_slots = 0;
+    PIC_HIP(ctx, hipMalloc(&ctx->thrust_part, (size_t)slots * 6 * sizeof(double)));
+    ctx->thrust_part_slots = slots;
+  }
+  if (is_f64(ctx) && !ctx->thrust_bprev)
+    PIC_HIP(ctx, hipMalloc(&ctx->thrust_bprev, 3 * (size_t)g.plane * esz));
+  if (ctx->diag_partial_cap < kDiagPartials) {
+    if (ctx->diag_partial) (void)hipFree(ctx->diag_partial);
+    ctx->diag_partial = nullptr;
+    ctx->diag_partial_cap = 0;
+    PIC_HIP(ctx, hipMalloc(&ctx->diag_partial, kDiagPartials * sizeof(double)));
+    ctx->diag_partial_cap = kDiagPartials;
+  }
+  k_fill_zero<<<1, 64, 0, ctx->stream>>>(ctx->thrust_acc, 16);
+  PIC_LAUNCH_CHECK(ctx);
+  ctx->thrust_on = 1;
+  return 0;
+}
+
+extern "C" int pic_thrust_ave_get(pic_ctx *ctx, double F[27], int64_t *nsteps,
+                                  int reset) {
+  if (!ctx || !F) return PIC_E_ARG;
+  if (!ctx->thrust_on) return set_error(ctx, PIC_E_STATE, "thrust_ave: not enabled");
+  int rc;
+  if (!is_f64(ctx) && (rc = f32::finish_deposit(ctx))) return rc;
+  if ((rc = thrust_settle(ctx))) return rc;
+  double a[12];
+  PIC_HIP(ctx, hipMemcpyAsync(a, ctx->thrust_acc, sizeof a, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // F[0..2] total Lorentz force, F[3..8] ion E and v x B parts, F[9..14]
+  // electron parts; F[15..26] (particles with flag < 0) stay 0: flag is 1
+  // for every live particle (:510, :6058)
+  for (int i = 0; i < 27; ++i) F[i] = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    F[3 + j] = a[j];
+    F[6 + j] = a[3 + j];
+    F[9 + j] = a[6 + j];
+    F[12 + j] = a[9 + j];
+    F[j] = ((a[j] + a[3 + j]) + a[6 + j]) + a[9 + j];
+  }
+  if (nsteps) *nsteps = ctx->thrust_steps;
+  if (reset) {
+    k_fill_zero<<<1, 64, 0, ctx->stream>>>(ctx->thrust_acc, 16);
+    PIC_LAUNCH_CHECK(ctx);
+    ctx->thrust_steps = 0;
+  }
   return 0;
 }
 
@@ -829,8 +992,15 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
       TRY(pic_sort(ctx, PIC_ELECTRON));
     }
     if (ctx->c_add > c.inflow_every - 1) {
+      // the inflow removes z < 4 dx (:6019-6021); where thrust_fave's box
+      // reaches below that (nz < 20) the pending force is taken first
+      if (ctx->thrust_pending && c.nz * 0.2 < 4.0) TRY(thrust_settle(ctx));
       TRY(pic_inflow(ctx));
       ctx->c_add = 0;
+    }
+    if (ctx->thrust_pending) {
+      TRY(thrust_keep_b(ctx));
+      ctx->thrust_now = 1;
     }
     TRY(pic_renew_b_half(ctx));
     TRY(pic_halo_b(ctx));
@@ -838,7 +1008,15 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
       timing_begin(ctx);
       rc = DISPATCH(push, ctx, sp, 1, 1, 1);
       timing_end(ctx, PIC_T_PUSH);
-      if (rc) return rc;
+      if (rc) {
+        ctx->thrust_now = 0;
+        return rc;
+      }
+    }
+    if (ctx->thrust_now) {
+      ctx->thrust_now = 0;
+      ctx->thrust_pending = 0;
+      ctx->thrust_steps++;
     }
     TRY(pic_renew_b_half(ctx));
     TRY(pic_halo_b(ctx));
@@ -850,6 +1028,7 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
     ctx->c_sort++;
     ctx->c_add++;
     ctx->step++;
+    if (ctx->thrust_on) ctx->thrust_pending = 1;
   }
 #undef TRY
   return 0;

@@ -220,6 +220,7 @@ extern "C" int pic_checkpoint_save(pic_ctx *ctx, const char *path) {
   for (int s = 0; s < 2; ++s) {
     h.n[s] = ctx->n[s];
     h.sorted_end[s] = ctx->sorted_end[s];
+    h.drift_halves[s] = f64 ? 0 : ctx->drift_halves[s];
   }
 
   Io io{ctx};
@@ -333,8 +334,12 @@ extern "C" int pic_checkpoint_load(pic_ctx *ctx, const char *path) {
   ctx->c_add = h.c_add;
   ctx->c_sort = h.c_sort;
   ctx->inflow_event = h.inflow_event;
-  for (int s = 0; s < 2; ++s)
+  for (int s = 0; s < 2; ++s) {
     ctx->sorted_end[s] = same_tiles && h.ntiles > 0 ? h.sorted_end[s] : 0;
+    // the windows of restored tiles follow the drift from where the writer's
+    // stood, so a resumed run deposits exactly as an uninterrupted one
+    ctx->drift_halves[s] = ctx->sorted_end[s] > 0 ? h.drift_halves[s] : 0;
+  }
   if (!f64) return f32::state_restored(ctx);
   return 0;
 }

@@ -62,7 +62,7 @@ __device__ inline void unpack(int32_t c, int &k, int &l, int &m) {
 struct Push32 {
   float T;          // q dt / (2 m), m signed
   float hdt;        // dt / (2 dx): offset change per unit velocity per half step
-  float c, c2;
+  float c, c2, inv_c;
   float qw;         // q w / dx^3: current per unit velocity
   float w;          // w / dx^3: density
   float eimf_x, eimf_y;  // V * IMF terms added to the gathered E (:2817-2818)
@@ -70,6 +70,9 @@ struct Push32 {
   int periodic[2];
   float fx;         // fixed-point units per unit value (a power of two)
   double fx_inv;    // 1 / fx
+  float box_lo[3], box_hi[3];  // thrust_fave's central box, cell units (open)
+  int box_margin;   // cells a particle can be from its tile's shifted anchor
+  uint32_t bprev;   // byte offset of the end-of-step total B from the bt planes
 };
 
 // grid of planes used by the pass
@@ -134,9 +137,13 @@ __device__ inline void boris32(const Push32 &p, const float f[6], float &vx,
   const float T = p.T;
   const float ex = f[0] + p.eimf_x, ey = f[1] + p.eimf_y, ez = f[2];
   const float d = p.c2 - (vx * vx + vy * vy + vz * vz);
-  const float g1 = d > 0.0f ? p.c * rsqrtf(d) : 0.0f;
+  // gamma1 = c / sqrt(c^2 - v^2); with u = gamma1 v, c^2 + u^2 = (gamma1 c)^2,
+  // so gamma2 = c / sqrt(c^2 + u^2) = 1 / gamma1 = sqrt(d) / c (1 when the
+  // reference's isnan clamp zeroes u)
+  const float r = rsqrtf(d);
+  const float g1 = d > 0.0f ? p.c * r : 0.0f;
   float ux = g1 * vx, uy = g1 * vy, uz = g1 * vz;
-  const float g2 = p.c * rsqrtf(p.c2 + ux * ux + uy * uy + uz * uz);
+  const float g2 = d > 0.0f ? (d * p.inv_c) * r : 1.0f;
   const float bx = g2 * f[3], by = g2 * f[4], bz = g2 * f[5];
   const float S = __fdividef(2.0f * T, 1.0f + T * T * (bx * bx + by * by + bz * bz));
   ux += ex * T;
@@ -154,17 +161,14 @@ __device__ inline void boris32(const Push32 &p, const float f[6], float &vx,
   vz = g * uz;
 }
 
-// half-step drift of one axis: offset += v dt/(2dx), at most one cell crossed
+// half-step drift of one axis: offset += v dt/(2dx), at most one cell
+// crossed (|v dt/(2dx)| < 1): the crossing is floor(offset) in {-1, 0, 1};
+// o - 1 is exact, o + 1 can round up to 1 and is kept below it
 __device__ inline void drift_axis(int &k, float &o, float v, float hdt) {
   o += v * hdt;
-  if (o >= 1.0f) {
-    o -= 1.0f;
-    k += 1;
-  } else if (o < 0.0f) {
-    o += 1.0f;
-    k -= 1;
-    if (o >= 1.0f) o = 0.99999994f;
-  }
+  const float f = floorf(o);
+  o = fminf(o - f, 0.99999994f);
+  k += (int)f;
 }
 
 // ---------------------------------------------------------------------------
@@ -255,6 +259,13 @@ __device__ inline Walk make_walk(int WY, int WZ) {
   return k;
 }
 
+// a contribution known to be non-negative (the density: weight and shape
+// factors are >= 0): its truncation is >= 0 and the high word is zero
+__device__ __forceinline__ void lds_fx_add_pos(char *acc, int byte, float v) {
+  __hip_atomic_fetch_add((PIC_LDS u64 *)(acc + byte), (u64)(uint32_t)(int)v,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ void lds_fx_add(char *acc, int byte, float v) {
   __hip_atomic_fetch_add((PIC_LDS u64 *)(acc + byte), to_fx(v), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -262,6 +273,7 @@ __device__ __forceinline__ void lds_fx_add(char *acc, int byte, float v) {
 
 // Deposit q * a_i b_j c_k at the 8 window nodes of a stencil whose lowest
 // node is at byte offset base; q carries the fixed-point scale.
+template <bool POS>
 __device__ __forceinline__ void lds_dep8(char *acc, int base, const Walk &k,
                                          float q, float a0, float a1, float b0,
                                          float b1, float c0, float c1) {
@@ -271,14 +283,17 @@ __device__ __forceinline__ void lds_dep8(char *acc, int base, const Walk &k,
   const float qa0 = q * A0, qa1 = q * A1;
   const float bc00 = B0 * C0, bc01 = B0 * C1, bc10 = B1 * C0, bc11 = B1 * C1;
   const int p = base + k.first;
-  lds_fx_add(acc, p + k.off[0], qa0 * bc00);
-  lds_fx_add(acc, p + k.off[1], qa0 * bc01);
-  lds_fx_add(acc, p + k.off[2], qa0 * bc10);
-  lds_fx_add(acc, p + k.off[3], qa0 * bc11);
-  lds_fx_add(acc, p + k.off[4], qa1 * bc00);
-  lds_fx_add(acc, p + k.off[5], qa1 * bc01);
-  lds_fx_add(acc, p + k.off[6], qa1 * bc10);
-  lds_fx_add(acc, p + k.off[7], qa1 * bc11);
+  const auto add = [&](int byte, float v) {
+    if (POS) lds_fx_add_pos(acc, byte, v); else lds_fx_add(acc, byte, v);
+  };
+  add(p + k.off[0], qa0 * bc00);
+  add(p + k.off[1], qa0 * bc01);
+  add(p + k.off[2], qa0 * bc10);
+  add(p + k.off[3], qa0 * bc11);
+  add(p + k.off[4], qa1 * bc00);
+  add(p + k.off[5], qa1 * bc01);
+  add(p + k.off[6], qa1 * bc10);
+  add(p + k.off[7], qa1 * bc11);
 }
 
 // the same stencil straight into the global planes (particles outside the
@@ -359,15 +374,15 @@ __device__ __forceinline__ void deposit_lds(const WinS &w, const Walk &wk, char 
   const int hx = X.bh != k ? sk : 0, hy = Y.bh != l ? sl : 0, hz = Z.bh != m ? 8 : 0;
   if (current) {
     const float qs = kp.qw * kp.fx;
-    lds_dep8(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, X.i0, X.i1, Y.h0, Y.h1,
-             Z.h0, Z.h1);
-    lds_dep8(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Y.i0, Y.i1,
-             Z.h0, Z.h1);
-    lds_dep8(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, X.h0, X.h1, Y.h0, Y.h1,
-             Z.i0, Z.i1);
+    lds_dep8<false>(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, X.i0, X.i1, Y.h0,
+                    Y.h1, Z.h0, Z.h1);
+    lds_dep8<false>(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Y.i0,
+                    Y.i1, Z.h0, Z.h1);
+    lds_dep8<false>(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, X.h0, X.h1, Y.h0,
+                    Y.h1, Z.i0, Z.i1);
   } else {
-    lds_dep8(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, X.h0, X.h1,
-             Y.h0, Y.h1, Z.h0, Z.h1);
+    lds_dep8<true>(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, X.h0, X.h1,
+                   Y.h0, Y.h1, Z.h0, Z.h1);
   }
 }
 
@@ -447,11 +462,11 @@ __device__ __forceinline__ void glob_coop(const Geom &g, bool pend, bool current
 // drift or gather + Boris, half drift, J deposit at x^{n+1/2}; DRIFT_N: hot
 // loop B (:818-867) - half drift, density deposit at x^{n+1}; BOUNDARY:
 // sr_particle (:2041-2531) on one rank.
-template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, typename A>
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, bool THRUST, typename A>
 __device__ __forceinline__ void particle_step(
     const Grid32 &G, const Push32 &kp, const WinS &w, const Walk &wk, char *acc,
     A *jx, A *jy, A *jz, A *nd, int32_t &cl, float &ox, float &oy, float &oz,
-    float &vx, float &vy, float &vz) {
+    float &vx, float &vy, float &vz, float (&th)[6], int box_class) {
   const Geom &g = G.g;
   if (cl < 0) return;
   int k, l, m;
@@ -467,15 +482,53 @@ __device__ __forceinline__ void particle_step(
   };
   if (PUSH_J) {
     const float xf = (float)k + ox, yf = (float)l + oy, zf = (float)m + oz;
-    if (inside() && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
-                      yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2])) {
+    const bool in_arr = inside();
+    const bool push = in_arr && !(xf < kp.blo[0] || xf > kp.bhi[0] || yf < kp.blo[1] ||
+                                  yf > kp.bhi[1] || zf < kp.blo[2] || zf > kp.bhi[2]);
+    // thrust_fave (:5124-5285) of the previous step: particles of the
+    // central box at their end-of-step positions and velocities, E of that
+    // step (unchanged since) and the total B of its end (bp planes)
+    // box_class (uniform): 0 every particle of the workgroup outside the
+    // box, 1 every one inside, 2 test each
+    bool box = false;
+    if (THRUST && box_class == 1) box = in_arr;
+    if (THRUST && box_class == 2)
+      box = in_arr && xf > kp.box_lo[0] && xf < kp.box_hi[0] && yf > kp.box_lo[1] &&
+            yf < kp.box_hi[1] && zf > kp.box_lo[2] && zf < kp.box_hi[2];
+    if (push || box) {
       const uint32_t sl = 4u * g.SZ, sk = 4u * g.SY * g.SZ;
       const uint32_t o = (((uint32_t)k * g.SY + l) * g.SZ + m) * 4u;
       const AxisW32 X = axis_w(k, ox), Y = axis_w(l, oy), Z = axis_w(m, oz);
+      const uint32_t hx = X.bh != k ? sk : 0u, hy = Y.bh != l ? sl : 0u,
+                     hz = Z.bh != m ? 4u : 0u;
       float f[6];
-      gather_o(G, o, sl, sk, X.bh != k ? sk : 0u, Y.bh != l ? sl : 0u,
-               Z.bh != m ? 4u : 0u, X, Y, Z, f);
-      boris32(kp, f, vx, vy, vz);
+      if (push) gather_o(G, o, sl, sk, hx, hy, hz, X, Y, Z, f);
+      if (THRUST && box) {
+        const uint32_t pox = o - hx, poy = o - hy, poz = o - hz;
+        float ex, ey, ez;
+        if (push) {
+          ex = f[0];
+          ey = f[1];
+          ez = f[2];
+        } else {
+          ex = lerp8o(G.ex, poy - hz, sl, sk, X.i0, X.i1, Y.h0, Y.h1, Z.h0, Z.h1);
+          ey = lerp8o(G.ey, pox - hz, sl, sk, X.h0, X.h1, Y.i0, Y.i1, Z.h0, Z.h1);
+          ez = lerp8o(G.ez, pox - hy, sl, sk, X.h0, X.h1, Y.h0, Y.h1, Z.i0, Z.i1);
+        }
+        // the kept planes follow the bt planes at a fixed byte offset
+        const float bx = lerp8o(G.btx, pox + kp.bprev, sl, sk, X.h0, X.h1, Y.i0, Y.i1, Z.i0, Z.i1);
+        const float by = lerp8o(G.bty, poy + kp.bprev, sl, sk, X.i0, X.i1, Y.h0, Y.h1, Z.i0, Z.i1);
+        const float bz = lerp8o(G.btz, poz + kp.bprev, sl, sk, X.i0, X.i1, Y.i0, Y.i1, Z.h0, Z.h1);
+        // q n dx^3 (E + v x B), split into the E and the v x B parts as
+        // F[3..8] / F[9..14]; the factor q n dx^3 is applied in FP64
+        th[0] += ex;
+        th[1] += ey;
+        th[2] += ez;
+        th[3] += vy * bz - vz * by;
+        th[4] += vz * bx - vx * bz;
+        th[5] += vx * by - vy * bx;
+      }
+      if (push) boris32(kp, f, vx, vy, vz);
     }
     drift_axis(k, ox, vx, kp.hdt);
     drift_axis(l, oy, vy, kp.hdt);
@@ -517,15 +570,35 @@ __device__ __forceinline__ void particle_step(
   cl = out;
 }
 
-template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET>
+// thrust_fave partials of one workgroup: the six per-lane fp32 sums added
+// in FP64 over the workgroup in a fixed order, stored at slot bid
+__device__ inline void thrust_store(const float (&th)[6], double *out, int bid) {
+  __shared__ double sh[6][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double a = th[q];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+    if (lane == 0) sh[q][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double a = sh[threadIdx.x][0];
+    for (int j = 1; j < kBlock / 64; ++j) a += sh[threadIdx.x][j];
+    out[(int64_t)bid * 6 + threadIdx.x] = a;
+  }
+}
+
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, bool THRUST>
 __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
     Grid32 G, TileGeom tg, Particles32 P, TileRun tr, Push32 kp,
     typename GlobT<DET>::T *jx, typename GlobT<DET>::T *jy,
-    typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd) {
+    typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd, double *thrust_part) {
   using A = typename GlobT<DET>::T;
   __shared__ u64 acc_mem[4 * kWin];
   __shared__ Win sw;
   __shared__ int64_t s_range[2];
+  __shared__ int s_box;
   const Geom &g = G.g;
   const int tid = threadIdx.x;
   const int bid = xcd_remap(blockIdx.x, tr.ntiles, (int)gridDim.x);
@@ -545,10 +618,38 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
     sw = w;
     s_range[0] = s;
     s_range[1] = e;
+    if (THRUST && thrust_part) {
+      // a tile's particles lie within box_margin cells of its shifted
+      // anchor box (cells [a0, a1] per axis), so the whole tile is inside
+      // the open box or outside it unless that range meets its faces;
+      // tail chunks test each particle
+      int cls = 2;
+      if (bid < tr.ntiles) {
+        const int tz = bid % tg.NTZ, ty = (bid / tg.NTZ) % tg.NTY,
+                  tx = bid / (tg.NTZ * tg.NTY);
+        const int T[3] = {tg.TX, tg.TY, tg.TZ}, t3[3] = {tx, ty, tz};
+        bool in = true, out = false;
+        for (int a = 0; a < 3; ++a) {
+          const int a0 = 2 + t3[a] * T[a] + tg.shift[a] - kp.box_margin;
+          const int a1 = 2 + t3[a] * T[a] + tg.shift[a] + T[a] - 1 + kp.box_margin;
+          in = in && (float)a0 > kp.box_lo[a] && (float)(a1 + 1) <= kp.box_hi[a];
+          out = out || (float)(a1 + 1) <= kp.box_lo[a] || (float)a0 >= kp.box_hi[a];
+        }
+        cls = out ? 0 : in ? 1 : 2;
+      }
+      s_box = cls;
+    } else {
+      s_box = 0;
+    }
   }
   __syncthreads();
   const int64_t s = s_range[0], e = s_range[1];
-  if (s >= e) return;
+  const int box_class = THRUST ? __builtin_amdgcn_readfirstlane(s_box) : 0;
+  float th[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (s >= e) {
+    if (THRUST && thrust_part && tid < 6) thrust_part[(int64_t)bid * 6 + tid] = 0.0;
+    return;
+  }
   // the window into scalar registers
   WinS w;
   w.on = __builtin_amdgcn_readfirstlane(sw.on ? 1 : 0) != 0;
@@ -584,15 +685,21 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
   // slot indices fit in int32 (the host bounds the stored count)
   const int s32 = (int)s, e32 = (int)e;
   const int q0 = s32 >> SH, q1 = (e32 + N - 1) >> SH;
+  // 32-bit byte offsets from the uniform array bases (SGPR-base loads and
+  // stores, no 64-bit address arithmetic per array)
+  const auto vat = [](auto *base, uint32_t byte) {
+    return (decltype(base))((char *)base + byte);
+  };
   for (int q = q0 + tid; q < q1; q += blockDim.x) {
     const int i0 = q << SH;
-    vi cs = *(const vi *)(P.cell + i0);
-    vf xs = *(const vf *)(P.ox + i0);
-    vf ys = *(const vf *)(P.oy + i0);
-    vf zs = *(const vf *)(P.oz + i0);
-    vf us = *(const vf *)(P.vx + i0);
-    vf vs = *(const vf *)(P.vy + i0);
-    vf ws = *(const vf *)(P.vz + i0);
+    const uint32_t bo = (uint32_t)i0 * 4u;
+    vi cs = *(const vi *)vat(P.cell, bo);
+    vf xs = *(const vf *)vat(P.ox, bo);
+    vf ys = *(const vf *)vat(P.oy, bo);
+    vf zs = *(const vf *)vat(P.oz, bo);
+    vf us = *(const vf *)vat(P.vx, bo);
+    vf vs = *(const vf *)vat(P.vy, bo);
+    vf ws = *(const vf *)vat(P.vz, bo);
     const bool full = i0 >= s32 && i0 + N - 1 < e32;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -600,8 +707,9 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
       if (i < s32 || i >= e32) continue;
       int c = cs[j];
       float x = xs[j], y = ys[j], z = zs[j], u = us[j], v = vs[j], ww = ws[j];
-      particle_step<PUSH_J, DRIFT_N, BOUNDARY, DET>(G, kp, w, wk, acc, jx, jy, jz,
-                                                    nd, c, x, y, z, u, v, ww);
+      particle_step<PUSH_J, DRIFT_N, BOUNDARY, DET, THRUST>(G, kp, w, wk, acc, jx, jy,
+                                                            jz, nd, c, x, y, z, u, v,
+                                                            ww, th, box_class);
       cs[j] = c;
       xs[j] = x;
       ys[j] = y;
@@ -611,13 +719,13 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
       ws[j] = ww;
     }
     if (full) {
-      *(vi *)(P.cell + i0) = cs;
-      *(vf *)(P.ox + i0) = xs;
-      *(vf *)(P.oy + i0) = ys;
-      *(vf *)(P.oz + i0) = zs;
-      *(vf *)(P.vx + i0) = us;
-      *(vf *)(P.vy + i0) = vs;
-      *(vf *)(P.vz + i0) = ws;
+      *(vi *)vat(P.cell, bo) = cs;
+      *(vf *)vat(P.ox, bo) = xs;
+      *(vf *)vat(P.oy, bo) = ys;
+      *(vf *)vat(P.oz, bo) = zs;
+      *(vf *)vat(P.vx, bo) = us;
+      *(vf *)vat(P.vy, bo) = vs;
+      *(vf *)vat(P.vz, bo) = ws;
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) {
@@ -633,6 +741,7 @@ __global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
       }
     }
   }
+  if (THRUST && thrust_part) thrust_store(th, thrust_part, bid);
   if (!w.on) return;
   __syncthreads();
   // flush the window: non-zero nodes inside the array go to the planes
@@ -1110,6 +1219,333 @@ __device__ inline void gather4(T *__restrict__ d, const T *__restrict__ s,
   *(V *)d = v;
 }
 
+
+// ---------------------------------------------------------------------------
+// tile sort (sorting :7475-7590 with the tile-major key), written for the
+// state a periodic sort meets: the particles are still tile-sorted from the
+// last sort except for a small appended tail, and most of them are still in
+// their tile.  The result is the stable sort by key, the same permutation
+// as a radix sort of (key, slot) pairs:
+//   1. each source block (a tile range of the last sort, or a tail chunk)
+//      counts the particles still in their tile and writes the others, in
+//      slot order, as one block per destination tile into that tile's
+//      mover list, recording (block's first slot, offset, count);
+//   2. a scan of (stayers + movers) per tile gives the new tile starts;
+//   3. one workgroup per tile orders its mover blocks by first slot, so the
+//      sequence (blocks from lower slots, its own stayers, blocks from
+//      higher slots) is in slot order, and places that sequence by a stable
+//      counting sort over the tile's cells.
+// Waves take contiguous quarters of a sequence: each counts, one scan sets
+// every (wave, value) start, and each wave then places its quarter in
+// order, ranking the lanes that share a value by ballots.  A block with
+// more than kDestMax destination tiles, a tile receiving more than
+// kMoverCap movers or kDescCap blocks, or a tile of more than kTileCells
+// cells takes the radix sort instead.
+constexpr int kMoverCap = 4096;
+constexpr int kDescCap = 64;
+constexpr int kDestMax = 64;
+constexpr int kTileCells = 1024;
+constexpr uint32_t kEmpty = 0xffffffffu;
+// tail particles per source block: the inflow overflow is appended tile by
+// tile, a few to some tens per slab tile, so a short block meets few tiles
+constexpr int kSortTail = 256;
+
+__device__ inline uint32_t tsort_key(const Geom &g, const TileGeom &tg, int32_t c,
+                                     uint32_t drop) {
+  if (c < 0) return drop;
+  int k, l, m;
+  unpack(c, k, l, m);
+  k -= 2;
+  l -= 2;
+  m -= 2;
+  if (k < 0 || k >= g.NX || l < 0 || l >= g.NY || m < 0 || m >= g.NZ) return drop;
+  return tg.key(k, l, m);
+}
+
+// rank of this lane among the active lanes (pred) of its wave holding the
+// same value v (nbits wide), and the number of such lanes
+__device__ inline int wave_rank(bool pred, uint32_t v, int nbits, int &count) {
+  const int lane = threadIdx.x & 63;
+  u64 same = __builtin_amdgcn_ballot_w64(pred);
+  for (int b = 0; b < nbits; ++b) {
+    const u64 on = __builtin_amdgcn_ballot_w64(pred && ((v >> b) & 1u));
+    same &= ((v >> b) & 1u) ? on : ~on;
+  }
+  count = __builtin_popcountll(same);
+  const u64 lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  return __builtin_popcountll(same & lt);
+}
+
+__global__ void k_tsort_clear(uint32_t *a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0u;
+}
+
+// pass 1: one workgroup per tile range of the last sort or tail chunk
+__global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRun tr,
+                                                     const int32_t *__restrict__ cell,
+                                                     uint32_t *stay, uint32_t *mcount,
+                                                     uint32_t *ndesc, uint4 *desc,
+                                                     uint2 *movers, uint32_t *flag) {
+  __shared__ uint32_t tkey[kDestMax];
+  __shared__ uint32_t wc[kBlock / 64][kDestMax];
+  __shared__ unsigned int kept_w[kBlock / 64];
+  const int bid = blockIdx.x, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  int64_t s, e;
+  if (bid < tr.ntiles) {
+    s = tr.tile_start[bid];
+    e = tr.tile_start[bid + 1];
+  } else {
+    s = tr.sorted_end + (int64_t)(bid - tr.ntiles) * kSortTail;
+    e = s + kSortTail < tr.n ? s + kSortTail : tr.n;
+  }
+  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
+  const uint32_t drop = (uint32_t)tg.ntiles() * cpt;
+  for (int h = tid; h < kDestMax; h += kBlock) {
+    tkey[h] = kEmpty;
+    for (int w = 0; w < kBlock / 64; ++w) wc[w][h] = 0u;
+  }
+  __syncthreads();
+  // this wave's contiguous quarter of the block
+  const int64_t len = e - s, per = (len + 3) / 4;
+  const int64_t q0 = s + wv * per, q1 = min(q0 + per, e);
+  // the destination's table slot (linear probing), -1 when the table is full
+  const auto slot_of = [&](uint32_t t, bool insert) -> int {
+    int h = (int)(t % kDestMax);
+    for (int i = 0; i < kDestMax; ++i) {
+      const uint32_t cur = tkey[h];
+      if (cur == t) return h;
+      if (cur == kEmpty) {
+        if (!insert) return -1;
+        const uint32_t old = atomicCAS(&tkey[h], kEmpty, t);
+        if (old == kEmpty || old == t) return h;
+      }
+      h = (h + 1) % kDestMax;
+    }
+    return -1;
+  };
+  unsigned int kept = 0;
+  for (int64_t i = q0 + lane; i < q1; i += 64) {
+    const uint32_t key = tsort_key(g, tg, cell[i], drop);
+    if (key == drop) continue;
+    const uint32_t t = key / cpt;
+    if (bid < tr.ntiles && t == (uint32_t)bid) {
+      ++kept;
+      continue;
+    }
+    const int h = slot_of(t, true);
+    if (h < 0) {
+      atomicOr(flag, 1u);
+      continue;
+    }
+    atomicAdd(&wc[wv][h], 1u);
+  }
+  for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
+  if (lane == 0) kept_w[wv] = kept;
+  __syncthreads();
+  // one reservation per destination: its total, its offset, one descriptor
+  if (tid < kDestMax && tkey[tid] != kEmpty) {
+    const uint32_t t = tkey[tid];
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += wc[w][tid];
+    const uint32_t off = atomicAdd(&mcount[t], tot);
+    const uint32_t k = atomicAdd(&ndesc[t], 1u);
+    if (k < (uint32_t)kDescCap)
+      desc[(int64_t)t * kDescCap + k] = make_uint4((uint32_t)s, off, tot, 0u);
+    if (k >= (uint32_t)kDescCap || off + tot > (uint32_t)kMoverCap) atomicOr(flag, 1u);
+    uint32_t run = off;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      const uint32_t c = wc[w][tid];
+      wc[w][tid] = run;
+      run += c;
+    }
+  }
+  if (tid == 0 && bid < tr.ntiles) {
+    unsigned int k = 0;
+    for (int w = 0; w < kBlock / 64; ++w) k += kept_w[w];
+    stay[bid] = k;
+  }
+  __syncthreads();
+  // each wave writes its movers in slot order
+  for (int64_t i0 = q0; i0 < q1; i0 += 64) {
+    const int64_t i = i0 + lane;
+    uint32_t key = drop, t = 0;
+    if (i < q1) key = tsort_key(g, tg, cell[i], drop);
+    if (key != drop) t = key / cpt;
+    int h = -1;
+    if (key != drop && !(bid < tr.ntiles && t == (uint32_t)bid)) h = slot_of(t, false);
+    const bool mv = h >= 0;
+    int cnt;
+    const int r = wave_rank(mv, (uint32_t)(mv ? h : 0), 6, cnt);
+    if (mv) {
+      const uint32_t pos = wc[wv][h] + (uint32_t)r;
+      if (pos < (uint32_t)kMoverCap)
+        movers[(int64_t)t * kMoverCap + pos] = make_uint2((uint32_t)i, key % cpt);
+    }
+    // the lanes' reads above precede this update in the wave's LDS order
+    if (mv && r == 0) wc[wv][h] += (uint32_t)cnt;
+  }
+}
+
+// pass 2: start[t] = exclusive scan of stay[t] + mcount[t] over the tiles,
+// start[nt] = the kept total (one workgroup of 1024 threads)
+__global__ __launch_bounds__(1024) void k_tsort_scan(const uint32_t *stay,
+                                                     const uint32_t *mcount, int nt,
+                                                     uint32_t *start) {
+  __shared__ uint32_t part[1024];
+  const int per = (nt + 1023) / 1024;
+  const int a = threadIdx.x * per, b = min(a + per, nt);
+  uint32_t sum = 0;
+  for (int i = a; i < b; ++i) sum += stay[i] + min(mcount[i], (uint32_t)kMoverCap);
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+    const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
+  for (int i = a; i < b; ++i) {
+    start[i] = run;
+    run += stay[i] + min(mcount[i], (uint32_t)kMoverCap);
+  }
+  if (threadIdx.x == 1023) start[nt] = part[1023];
+}
+
+// pass 3: one workgroup per tile t; writes idx[start[t] + position] = slot
+__global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
+                                                     const uint32_t *__restrict__ old_start,
+                                                     const int32_t *__restrict__ cell,
+                                                     const uint32_t *__restrict__ ndesc,
+                                                     const uint4 *__restrict__ desc,
+                                                     const uint2 *__restrict__ movers,
+                                                     const uint32_t *__restrict__ start,
+                                                     uint32_t *idx) {
+  // the sequence as segments: first sequence index, source offset, kind
+  // (0: movers at that offset of the list, 1: the range from that slot)
+  __shared__ uint32_t seg_q[kDescCap + 2], seg_src[kDescCap + 2];
+  __shared__ int seg_kind[kDescCap + 2];
+  __shared__ int s_nseg;
+  __shared__ uint4 d[kDescCap];
+  __shared__ uint32_t cnt[kBlock / 64][kTileCells];
+  __shared__ uint32_t tot[kTileCells];
+  __shared__ uint32_t part[kBlock];
+  const int t = blockIdx.x, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
+  const uint32_t drop = (uint32_t)tg.ntiles() * cpt;
+  const uint32_t s0 = old_start[t], len = old_start[t + 1] - s0;
+  if (tid == 0) {
+    // mover blocks by first slot (insertion sort of at most kDescCap), the
+    // own range spliced in at its first slot
+    const int nd = (int)min(ndesc[t], (uint32_t)kDescCap);
+    for (int i = 0; i < nd; ++i) {
+      uint4 x = desc[(int64_t)t * kDescCap + i];
+      int j = i;
+      while (j > 0 && d[j - 1].x > x.x) {
+        d[j] = d[j - 1];
+        --j;
+      }
+      d[j] = x;
+    }
+    uint32_t q = 0;
+    int ns = 0;
+    bool own = false;
+    for (int i = 0; i <= nd; ++i) {
+      if (!own && (i == nd || d[i].x > s0)) {
+        seg_q[ns] = q;
+        seg_src[ns] = s0;
+        seg_kind[ns++] = 1;
+        q += len;
+        own = true;
+      }
+      if (i == nd) break;
+      seg_q[ns] = q;
+      seg_src[ns] = d[i].y;
+      seg_kind[ns++] = 0;
+      // within the list (a block past its end overflowed: the result is
+      // then discarded, but every read stays inside the list)
+      const uint32_t off = min(d[i].y, (uint32_t)kMoverCap);
+      q += min(d[i].z, (uint32_t)kMoverCap - off);
+    }
+    seg_q[ns] = q;  // end of the sequence
+    s_nseg = ns;
+  }
+  for (int j = tid; j < (int)cpt; j += kBlock)
+    for (int w = 0; w < kBlock / 64; ++w) cnt[w][j] = 0u;
+  __syncthreads();
+  const int ns = s_nseg;
+  const uint32_t total = seg_q[ns];
+  // element q of the sequence: (slot, cell in the tile), false for a range
+  // slot whose particle left the tile or dropped out
+  const auto elem = [&](uint32_t q, uint32_t &slot, uint32_t &c) -> bool {
+    int lo = 0, hi = ns - 1;  // last segment starting at or before q
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (seg_q[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    const uint32_t r = q - seg_q[lo];
+    if (seg_kind[lo] == 0) {
+      const uint2 m = movers[(int64_t)t * kMoverCap + seg_src[lo] + r];
+      slot = m.x;
+      c = m.y;
+      return true;
+    }
+    slot = seg_src[lo] + r;
+    const uint32_t key = tsort_key(g, tg, cell[slot], drop);
+    if (key == drop || key / cpt != (uint32_t)t) return false;
+    c = key % cpt;
+    return true;
+  };
+  const uint32_t per = (total + 3) / 4;
+  const uint32_t q0 = min(wv * per, total), q1 = min(q0 + per, total);
+  for (uint32_t q = q0 + lane; q < q1; q += 64) {
+    uint32_t slot, c;
+    if (elem(q, slot, c)) atomicAdd(&cnt[wv][c], 1u);
+  }
+  __syncthreads();
+  // cell starts (exclusive scan over the cells' totals), then each wave's
+  // start per cell
+  for (uint32_t c = tid; c < cpt; c += kBlock) {
+    uint32_t v = 0;
+    for (int w = 0; w < kBlock / 64; ++w) v += cnt[w][c];
+    tot[c] = v;
+  }
+  __syncthreads();
+  const uint32_t cper = (cpt + kBlock - 1) / kBlock;
+  const uint32_t ca = min(tid * cper, cpt), cb = min(ca + cper, cpt);
+  uint32_t sum = 0;
+  for (uint32_t c = ca; c < cb; ++c) sum += tot[c];
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {  // inclusive Hillis-Steele scan
+    const uint32_t v = tid >= o ? part[tid - o] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = start[t] + (tid > 0 ? part[tid - 1] : 0u);
+  for (uint32_t c = ca; c < cb; ++c) {
+    for (int w = 0; w < kBlock / 64; ++w) {
+      const uint32_t v = cnt[w][c];
+      cnt[w][c] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int nbits = 1;
+  while ((1u << nbits) < cpt) ++nbits;
+  for (uint32_t qb = q0; qb < q1; qb += 64) {
+    const uint32_t q = qb + lane;
+    uint32_t slot = 0, c = 0;
+    const bool ok = q < q1 && elem(q, slot, c);
+    int n;
+    const int r = wave_rank(ok, c, nbits, n);
+    if (ok) idx[cnt[wv][c] + (uint32_t)r] = slot;
+    if (ok && r == 0) cnt[wv][c] += (uint32_t)n;
+  }
+}
+
 __global__ void k_gather32(Particles32 dst, Particles32 src,
                            const uint32_t *__restrict__ idx, int64_t n) {
   const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
@@ -1297,9 +1733,16 @@ __global__ __launch_bounds__(256) void k_diag_particles32(
 struct State32 {
   float *sig32 = nullptr;          // 6 x stride floats
   unsigned long long *fx = nullptr;  // deterministic accumulators, 8 planes
-  float *bt = nullptr;               // total B = B0 + B1, 3 planes
+  float *bt = nullptr;               // total B = B0 + B1, 3 planes (+ 3 kept)
   bool fx_dirty = false;
   double fx_scale = 0;
+  // tile sort: per tile stayers, mover and block counts, block descriptors,
+  // mover lists and new starts; overflow flag
+  uint32_t *ts_stay = nullptr, *ts_mcount = nullptr, *ts_ndesc = nullptr;
+  uint4 *ts_desc = nullptr;
+  uint2 *ts_movers = nullptr;
+  uint32_t *ts_start = nullptr, *ts_flag = nullptr;
+  int ts_ntiles = 0;
 };
 
 static State32 *st(pic_ctx *ctx) { return (State32 *)ctx->form_state; }
@@ -1379,7 +1822,9 @@ int init_state(pic_ctx *ctx) {
     // power of two with |contribution| * scale <= 2^30
     S->fx_scale = ldexp(1.0, (int)floor(30.0 - log2(unit)));
   }
-  PIC_HIP(ctx, hipMalloc(&S->bt, (size_t)3 * ctx->geom.plane * sizeof(float)));
+  // 6 planes: the total B of the gather, then the end-of-step copy that
+  // thrust_fave's fused force reads at a fixed offset
+  PIC_HIP(ctx, hipMalloc(&S->bt, (size_t)6 * ctx->geom.plane * sizeof(float)));
   {
     int rc = refresh_total_b(ctx);
     if (rc) return rc;
@@ -1397,6 +1842,9 @@ void free_state(pic_ctx *ctx) {
   if (S->sig32) (void)hipFree(S->sig32);
   if (S->fx) (void)hipFree(S->fx);
   if (S->bt) (void)hipFree(S->bt);
+  for (void *p : {(void *)S->ts_stay, (void *)S->ts_mcount, (void *)S->ts_desc,
+                  (void *)S->ts_movers, (void *)S->ts_start, (void *)S->ts_flag})
+    if (p) (void)hipFree(p);
   delete S;
   ctx->form_state = nullptr;
 }
@@ -1513,6 +1961,7 @@ static Push32 push_const(const pic_ctx *ctx, int sp) {
   p.hdt = (float)(0.5 * c.dt / c.dx);
   p.c = (float)c.c;
   p.c2 = (float)(c.c * c.c);
+  p.inv_c = (float)(1.0 / c.c);
   const double dx3 = c.dx * c.dx * c.dx;
   p.qw = (float)(c.q * c.weight[sp] / dx3);
   p.w = (float)(c.weight[sp] / dx3);
@@ -1527,6 +1976,15 @@ static Push32 push_const(const pic_ctx *ctx, int sp) {
   const double sc = ctx->form_state ? ((State32 *)ctx->form_state)->fx_scale : 1.0;
   p.fx = (float)sc;
   p.fx_inv = 1.0 / sc;
+  // thrust_fave's box (:5148-5150): 0.2 N dx < x < 0.8 N dx on each axis;
+  // the margin and the kept planes' offset are set when the sum is on
+  p.box_margin = 0;
+  p.bprev = 0;
+  const int N[3] = {c.nx, c.ny, c.nz};
+  for (int a = 0; a < 3; ++a) {
+    p.box_lo[a] = (float)(N[a] * 0.2);
+    p.box_hi[a] = (float)(N[a] * 0.8);
+  }
   return p;
 }
 
@@ -1534,10 +1992,11 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
          int do_boundary) {
   const int64_t n = ctx->n[sp];
   if (n == 0) return 0;
-  if (n > (int64_t)INT32_MAX - 8)
+  // slot indices and byte offsets of the pass are 32-bit
+  if (n > ((int64_t)1 << 30) - 8)
     return set_error(ctx, PIC_E_CAPACITY,
-                     "fp32 particle pass: %lld stored particles exceed the "
-                     "32-bit slot index", (long long)n);
+                     "fp32 particle pass: %lld stored particles of one species "
+                     "exceed 2^30 (32-bit byte offsets)", (long long)n);
   State32 *S = st(ctx);
   TileGeom tg = tile_geom(ctx);
   add_drift_shift(tg, ctx->cfg, ctx->drift_halves[sp]);
@@ -1556,6 +2015,32 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
   Grid32 G = grid32(ctx);
   const bool det = S->fx != nullptr;
   const int sel = (do_push_j ? 4 : 0) | (do_drift_n ? 2 : 0) | (do_boundary ? 1 : 0);
+  // The fused pass always runs the instantiation that can add the previous
+  // step's thrust_fave (its push and deposit arithmetic is then the same
+  // whether or not the sum is on: the unit contracts floating-point
+  // operations freely, and two instantiations could round differently);
+  // without a partials buffer it adds nothing.
+  const bool thrust = ctx->thrust_now && sel == 7;
+  double *tp = nullptr;
+  if (thrust) {
+    if ((int64_t)nb > ctx->thrust_part_slots)
+      return set_error(ctx, PIC_E_CAPACITY, "thrust: %u workgroups, %lld partial slots",
+                       nb, (long long)ctx->thrust_part_slots);
+    tp = ctx->thrust_part;
+    // the kept end-of-step B is the upper half of the 6-plane bt block
+    kp.bprev = (uint32_t)(3 * ctx->geom.plane * sizeof(float));
+    // farthest a particle moved from its tile since the sort, beyond the
+    // window's drift shift: |v| < c per half step, h halves before this pass
+    const pic_config &c = ctx->cfg;
+    const int h = ctx->drift_halves[sp] - 2;  // incremented above for this pass
+    int smax = 0;
+    for (int a = 0; a < 3; ++a) smax = std::max(smax, std::abs(tg.shift[a]));
+    kp.box_margin = (int)ceil(h * 0.5 * c.c * c.dt / c.dx) + smax + 1;
+    // a particle that wrapped across a periodic face lies within the margin
+    // of the opposite face; only while that stays outside the box may whole
+    // tiles be classified
+    if (kp.box_margin + 2 >= 0.2 * std::min(c.nx, c.ny)) kp.box_margin = 1 << 20;
+  }
   if (det) {
     unsigned long long *base = S->fx;
     const int64_t pl = ctx->geom.plane;
@@ -1564,9 +2049,12 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
                        *jz = base + (off + 2) * pl,
                        *nd = base + (6 + (sp == PIC_ION ? 0 : 1)) * pl;
 #define L(A, B, C) \
-  k_particles32<A, B, C, true><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd)
-    switch (sel) {
-      case 7: L(true, true, true); break;
+  k_particles32<A, B, C, true, false><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
+    switch (sel == 7 ? 8 : sel) {
+      case 8:
+        k_particles32<true, true, true, true, true>
+            <<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
+        break;
       case 6: L(true, true, false); break;
       case 4: L(true, false, false); break;
       case 2: L(false, true, false); break;
@@ -1584,9 +2072,12 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
     float *jz = F.f[sp == PIC_ION ? PIC_JIZ : PIC_JEZ];
     float *nd = F.f[sp == PIC_ION ? PIC_NI : PIC_NE];
 #define L(A, B, C) \
-  k_particles32<A, B, C, false><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd)
-    switch (sel) {
-      case 7: L(true, true, true); break;
+  k_particles32<A, B, C, false, false><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
+    switch (sel == 7 ? 8 : sel) {
+      case 8:
+        k_particles32<true, true, true, false, true>
+            <<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
+        break;
       case 6: L(true, true, false); break;
       case 4: L(true, false, false); break;
       case 2: L(false, true, false); break;
@@ -1598,10 +2089,17 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
 #undef L
   }
   PIC_LAUNCH_CHECK(ctx);
+  if (thrust) {
+    const pic_config &c = ctx->cfg;
+    const double qs = sp == PIC_ION ? c.q : -c.q;
+    return thrust_add(ctx, sp, (int)nb, qs * c.weight[sp] * c.dx * c.dx * c.dx);
+  }
   return 0;
 }
 
 int particle_boundary(pic_ctx *ctx, int sp) { return push(ctx, sp, 0, 0, 1); }
+
+float *total_b(pic_ctx *ctx) { return st(ctx)->bt; }
 
 // stable compaction of live particles (tile ranges become invalid)
 static int compact(pic_ctx *ctx, int sp) {
@@ -1648,9 +2146,12 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   const TileGeom tg = tile_geom(ctx);
   const int nslab = nface * tg.NTX * tg.NTY;
   // in-tile refill: tiles current, the slab layers inside the end z tiles,
-  // slots kept by the adjoint's forward (it needs the batch contiguous)
+  // slots kept by the adjoint's forward (it needs the batch contiguous).
+  // With one z tile layer both faces' slabs lie in the same tiles, and two
+  // workgroups would fill the same dead slots: such tilings append.
   const bool in_tile = add > 0 && !ctx->adj_nosort && ctx->sorted_end[sp] > 0 &&
                        ctx->sorted_end[sp] <= ctx->n[sp] && tg.ntiles() > 1 &&
+                       (nface == 1 || tg.NTZ >= 2) &&
                        tg.TZ >= 2 && g.NZ % tg.TZ == 0 && g.NX % tg.TX == 0 &&
                        g.NY % tg.TY == 0 && (int64_t)nslab + 1 <= ctx->cap &&
                        (int64_t)tg.TX * tg.TY * 2 * c.ppc * sizeof(int) <= 65536;
@@ -1694,6 +2195,68 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   return 0;
 }
 
+// The tile sort (passes above) when the tiles of the last sort are current;
+// *done is false when a tile overflowed its mover list (the caller then
+// takes the radix sort).  Leaves the permutation in ctx->idx, the new tile
+// starts in ctx->tile_start[sp] and the kept count in *valid.
+static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
+                     int64_t *valid) {
+  State32 *S = st(ctx);
+  const Geom &g = ctx->geom;
+  const int nt = tg.ntiles();
+  const int64_t n = ctx->n[sp];
+  *done = false;
+  if (S->ts_ntiles < nt) {
+    for (void **p : {(void **)&S->ts_stay, (void **)&S->ts_mcount, (void **)&S->ts_ndesc,
+                     (void **)&S->ts_desc, (void **)&S->ts_movers, (void **)&S->ts_start,
+                     (void **)&S->ts_flag}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    S->ts_ntiles = 0;
+    PIC_HIP(ctx, hipMalloc(&S->ts_stay, (size_t)nt * sizeof(uint32_t)));
+    PIC_HIP(ctx, hipMalloc(&S->ts_mcount, (size_t)2 * nt * sizeof(uint32_t)));
+    S->ts_ndesc = S->ts_mcount + nt;
+    PIC_HIP(ctx, hipMalloc(&S->ts_desc, (size_t)nt * kDescCap * sizeof(uint4)));
+    PIC_HIP(ctx, hipMalloc(&S->ts_movers, (size_t)nt * kMoverCap * sizeof(uint2)));
+    PIC_HIP(ctx, hipMalloc(&S->ts_start, (size_t)(nt + 1) * sizeof(uint32_t)));
+    PIC_HIP(ctx, hipMalloc(&S->ts_flag, sizeof(uint32_t)));
+    S->ts_ntiles = nt;
+  }
+  TileRun tr;
+  tr.tile_start = ctx->tile_start[sp];
+  tr.sorted_end = ctx->sorted_end[sp];
+  tr.ntiles = nt;
+  tr.n = n;
+  tr.ntail = (int)div_up(n - tr.sorted_end, kSortTail);
+  const Particles32 src = p32(ctx, sp, ctx->cur[sp]);
+  // mover counts and descriptor counts (adjacent), and the flag
+  k_tsort_clear<<<(unsigned)div_up(2 * nt, kBlock), kBlock, 0, ctx->stream>>>(S->ts_mcount,
+                                                                         2 * nt);
+  k_tsort_clear<<<1, 64, 0, ctx->stream>>>(S->ts_flag, 1);
+  k_tsort_split<<<(unsigned)(nt + tr.ntail), kBlock, 0, ctx->stream>>>(
+      g, tg, tr, src.cell, S->ts_stay, S->ts_mcount, S->ts_ndesc, S->ts_desc,
+      S->ts_movers, S->ts_flag);
+  k_tsort_scan<<<1, 1024, 0, ctx->stream>>>(S->ts_stay, S->ts_mcount, nt, S->ts_start);
+  k_tsort_place<<<(unsigned)nt, kBlock, 0, ctx->stream>>>(
+      g, tg, ctx->tile_start[sp], src.cell, S->ts_ndesc, S->ts_desc, S->ts_movers,
+      S->ts_start, ctx->idx);
+  PIC_LAUNCH_CHECK(ctx);
+  uint32_t *hc = (uint32_t *)ctx->host_counter;
+  PIC_HIP(ctx, hipMemcpyAsync(hc, S->ts_start + nt, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  PIC_HIP(ctx, hipMemcpyAsync(hc + 1, S->ts_flag, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (hc[1] != 0u) return 0;  // a list or a table overflowed: radix sort
+  PIC_HIP(ctx, hipMemcpyAsync(ctx->tile_start[sp], S->ts_start,
+                              (size_t)(nt + 1) * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  *valid = (int64_t)hc[0];
+  *done = true;
+  return 0;
+}
+
 int sort(pic_ctx *ctx, int sp) {
   const Geom &g = ctx->geom;
   State32 *S = st(ctx);
@@ -1706,6 +2269,26 @@ int sort(pic_ctx *ctx, int sp) {
     return 0;
   }
   Particles32 src = p32(ctx, sp, ctx->cur[sp]);
+  if (nt > 1 && tg.TX * tg.TY * tg.TZ <= kTileCells && ctx->sorted_end[sp] > 0 &&
+      ctx->sorted_end[sp] <= n && !ctx->adj_nosort) {
+    bool done = false;
+    int64_t valid = 0;
+    int rc = tile_sort(ctx, sp, tg, &done, &valid);
+    if (rc) return rc;
+    if (done) {
+      Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
+      if (valid > 0)
+        k_gather32<<<(unsigned)div_up(valid, 4 * kBlock), kBlock, 0, ctx->stream>>>(
+            dst, src, ctx->idx, valid);
+      PIC_LAUNCH_CHECK(ctx);
+      ctx->cur[sp] = 1 - ctx->cur[sp];
+      ctx->n[sp] = valid;
+      ctx->sorted_end[sp] = valid;
+      ctx->sort_tile_count++;
+      return 0;
+    }
+  }
+  ctx->sort_radix_count++;
   k_sort_keys32<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->stream>>>(
       g, tg, src.cell, n, ctx->keys);
   PIC_LAUNCH_CHECK(ctx);
@@ -1751,11 +2334,36 @@ int count_live(pic_ctx *ctx, int sp, int64_t *out) {
   return 0;
 }
 
-int diagnose(pic_ctx *ctx, pic_diag *out) {
+static_assert(kNQF == kDiagNQF && kNQP == kDiagNQP, "diagnostics layout");
+
+int diagnose_partials(pic_ctx *ctx, double *pf, unsigned long long *counts) {
   const pic_config &c = ctx->cfg;
   const Geom &g = ctx->geom;
-  const int nbf = 1024, nbp = 2048;
-  const int64_t need = (int64_t)nbf * kNQF + 2 * (int64_t)nbp * kNQP;
+  double *pp[2] = {pf + (int64_t)kDiagNBF * kNQF,
+                   pf + (int64_t)kDiagNBF * kNQF + (int64_t)kDiagNBP * kNQP};
+  k_diag_fields32<<<kDiagNBF, kBlock, 0, ctx->stream>>>(g, fields<float>(ctx), c.dx,
+                                                        c.eps0, c.mu0, c.q, pf);
+  PIC_LAUNCH_CHECK(ctx);
+  Grid32 G = grid32(ctx);
+  for (int sp = 0; sp < 2; ++sp) {
+    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+    k_diag_particles32<<<kDiagNBP, kBlock, 0, ctx->stream>>>(
+        G, P, ctx->n[sp], c.dx, c.c, c.mass[sp], c.weight[sp],
+        sp == PIC_ION ? c.q : -c.q, pp[sp]);
+    PIC_LAUNCH_CHECK(ctx);
+    if (counts && ctx->n[sp] > 0) {
+      k_count_live32<<<(unsigned)std::min<int64_t>(2048, div_up(ctx->n[sp], kBlock)),
+                       kBlock, 0, ctx->stream>>>(P.cell, ctx->n[sp],
+                                                 (int64_t *)(counts + sp));
+      PIC_LAUNCH_CHECK(ctx);
+    }
+  }
+  return 0;
+}
+
+int diagnose(pic_ctx *ctx, pic_diag *out) {
+  const int nbf = kDiagNBF, nbp = kDiagNBP;
+  const int64_t need = kDiagPartials;
   if (ctx->diag_partial_cap < need) {
     if (ctx->diag_partial) (void)hipFree(ctx->diag_partial);
     ctx->diag_partial = nullptr;
@@ -1765,17 +2373,8 @@ int diagnose(pic_ctx *ctx, pic_diag *out) {
   double *pf = ctx->diag_partial;
   double *pp[2] = {pf + (int64_t)nbf * kNQF,
                    pf + (int64_t)nbf * kNQF + (int64_t)nbp * kNQP};
-  k_diag_fields32<<<nbf, kBlock, 0, ctx->stream>>>(g, fields<float>(ctx), c.dx,
-                                                   c.eps0, c.mu0, c.q, pf);
-  PIC_LAUNCH_CHECK(ctx);
-  Grid32 G = grid32(ctx);
-  for (int sp = 0; sp < 2; ++sp) {
-    Particles32 P = p32(ctx, sp, ctx->cur[sp]);
-    k_diag_particles32<<<nbp, kBlock, 0, ctx->stream>>>(
-        G, P, ctx->n[sp], c.dx, c.c, c.mass[sp], c.weight[sp],
-        sp == PIC_ION ? c.q : -c.q, pp[sp]);
-    PIC_LAUNCH_CHECK(ctx);
-  }
+  int rc0 = diagnose_partials(ctx, pf, nullptr);
+  if (rc0) return rc0;
   double *h = (double *)malloc(need * sizeof(double));
   if (!h) return set_error(ctx, PIC_E_NOMEM, "diagnose: host buffer");
   hipError_t e = hipMemcpyAsync(h, pf, need * sizeof(double),

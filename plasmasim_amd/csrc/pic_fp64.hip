@@ -542,10 +542,34 @@ __device__ inline void dep8_64(double *lds, double *glob, bool in_lds,
   }
 }
 
-template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY>
+// thrust_fave partials of one workgroup (six per-thread sums added over the
+// workgroup in a fixed order), stored at slot bid
+__device__ inline void thrust_store64(const double (&th)[6], double *out, int bid) {
+  __shared__ double sh[6][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double a = th[q];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+    if (lane == 0) sh[q][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double a = sh[threadIdx.x][0];
+    for (int j = 1; j < kBlock / 64; ++j) a += sh[threadIdx.x][j];
+    out[(int64_t)bid * 6 + threadIdx.x] = a;
+  }
+}
+
+// THRUST: also add thrust_fave (:5124-5285) of the previous step for the
+// particles of the central box: their positions and velocities at the start
+// of the pass are that step's end state, E is unchanged since, and Fp holds
+// the total B of its end (E planes shared with F).
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool THRUST>
 __global__ __launch_bounds__(256) void k_particles_tiled(
     Geom g, TileGeom tg, Fields<double> F, Particles64 P, TileRun tr,
-    StepConst kc, double *jx, double *jy, double *jz, double *nd) {
+    StepConst kc, double *jx, double *jy, double *jz, double *nd,
+    Fields<double> Fp, double *thrust_part) {
   __shared__ double acc[4 * kWin64];
   __shared__ Win sw;
   __shared__ int64_t s_range[2];
@@ -572,7 +596,11 @@ __global__ __launch_bounds__(256) void k_particles_tiled(
   }
   __syncthreads();
   const int64_t s = s_range[0], e = s_range[1];
-  if (s >= e) return;
+  double th[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (s >= e) {
+    if (THRUST && tid < 6) thrust_part[(int64_t)bid * 6 + tid] = 0.0;
+    return;
+  }
   const Win w = sw;
   const int WN = w.on ? w.WX * w.WY * w.WZ : 0;
   for (int i = tid; i < 4 * WN; i += blockDim.x) acc[(i / WN) * kWin64 + i % WN] = 0.0;
@@ -586,6 +614,18 @@ __global__ __launch_bounds__(256) void k_particles_tiled(
     const bool safe0 = in_array(g, x, y, z, dx);
     const bool lds_ok =
         safe0 && in_win(w, (int)(x / dx), (int)(y / dx), (int)(z / dx));
+    if (THRUST && safe0 && x < g.NX * dx * 0.8 && x > g.NX * dx * 0.2 &&
+        y < g.NY * dx * 0.8 && y > g.NY * dx * 0.2 && z < g.NZ * dx * 0.8 &&
+        z > g.NZ * dx * 0.2) {
+      double f[6];
+      gather(Fp, g, x, y, z, dx, f);
+      th[0] += f[0];
+      th[1] += f[1];
+      th[2] += f[2];
+      th[3] += vy * f[5] - vz * f[4];
+      th[4] += vz * f[3] - vx * f[5];
+      th[5] += vx * f[4] - vy * f[3];
+    }
     if (PUSH_J) {
       if (x < kc.blo[0] || x > kc.bhi[0] || y < kc.blo[1] || y > kc.bhi[1] ||
           z < kc.blo[2] || z > kc.bhi[2] || !safe0) {
@@ -651,6 +691,7 @@ __global__ __launch_bounds__(256) void k_particles_tiled(
     P.vy[i] = vy;
     P.vz[i] = vz;
   }
+  if (THRUST) thrust_store64(th, thrust_part, bid);
   if (!w.on) return;
   __syncthreads();
   const int WYZ = w.WY * w.WZ;
@@ -1011,10 +1052,26 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
   tr.ntail = (int)div_up(n - tr.sorted_end, kTailChunk);
   const unsigned nb = (unsigned)(tr.ntiles + tr.ntail);
   const int sel = (do_push_j ? 4 : 0) | (do_drift_n ? 2 : 0) | (do_boundary ? 1 : 0);
+  // the fused pass of pic_step adds the previous step's thrust_fave
+  const bool thrust = ctx->thrust_now && sel == 7;
+  Fields<double> Fp = F;
+  double *tp = nullptr;
+  if (thrust) {
+    if ((int64_t)nb > ctx->thrust_part_slots)
+      return set_error(ctx, PIC_E_CAPACITY, "thrust: %u workgroups, %lld partial slots",
+                       nb, (long long)ctx->thrust_part_slots);
+    const double *bp = (const double *)ctx->thrust_bprev;
+    for (int q = 0; q < 3; ++q) Fp.f[PIC_BX + q] = (double *)bp + q * ctx->geom.plane;
+    tp = ctx->thrust_part;
+  }
 #define LAUNCH(A, B, C)                                                    \
-  k_particles_tiled<A, B, C><<<nb, kBlock, 0, ctx->stream>>>(               \
-      ctx->geom, tg, F, P, tr, kc, jx, jy, jz, nd)
-  switch (sel) {
+  k_particles_tiled<A, B, C, false><<<nb, kBlock, 0, ctx->stream>>>(        \
+      ctx->geom, tg, F, P, tr, kc, jx, jy, jz, nd, Fp, tp)
+  switch (thrust ? 8 : sel) {
+    case 8:
+      k_particles_tiled<true, true, true, true><<<nb, kBlock, 0, ctx->stream>>>(
+          ctx->geom, tg, F, P, tr, kc, jx, jy, jz, nd, Fp, tp);
+      break;
     case 7: LAUNCH(true, true, true); break;
     case 6: LAUNCH(true, true, false); break;
     case 4: LAUNCH(true, false, false); break;
@@ -1026,6 +1083,11 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
   }
 #undef LAUNCH
   PIC_LAUNCH_CHECK(ctx);
+  if (thrust) {
+    const pic_config &c = ctx->cfg;
+    const double qs = sp == PIC_ION ? c.q : -c.q;
+    return thrust_add(ctx, sp, (int)nb, qs * c.weight[sp] * c.dx * c.dx * c.dx);
+  }
   return 0;
 }
 
@@ -1098,6 +1160,7 @@ int sort(pic_ctx *ctx, int sp) {
   const Geom &g = ctx->geom;
   const int64_t n = ctx->n[sp];
   if (n == 0) return 0;
+  ctx->sort_radix_count++;
   Particles64 src = p64(ctx, sp, ctx->cur[sp]);
   const TileGeom tg = tile_geom(ctx);
   const int nt = tg.ntiles();
@@ -1146,13 +1209,39 @@ int count_live(pic_ctx *ctx, int sp, int64_t *out) {
   return 0;
 }
 
-int diagnose(pic_ctx *ctx, pic_diag *out) {
+static_assert(kDiagField + kDiagBox == kDiagNQF && kDiagPart == kDiagNQP,
+              "diagnostics layout");
+
+int diagnose_partials(pic_ctx *ctx, double *pf, unsigned long long *counts) {
   const pic_config &c = ctx->cfg;
   const Geom &g = ctx->geom;
   Fields<double> F = fields<double>(ctx);
-  const int nbf = 1024, nbp = 2048;
-  constexpr int NQF = kDiagField + kDiagBox, NQP = kDiagPart;
-  const int64_t need = (int64_t)nbf * NQF + 2 * (int64_t)nbp * NQP;
+  double *pp[2] = {pf + (int64_t)kDiagNBF * kDiagNQF,
+                   pf + (int64_t)kDiagNBF * kDiagNQF + (int64_t)kDiagNBP * kDiagNQP};
+  k_diag_fields<<<kDiagNBF, kBlock, 0, ctx->stream>>>(g, F, c.dx, c.eps0, c.mu0,
+                                                      c.q, pf);
+  PIC_LAUNCH_CHECK(ctx);
+  for (int sp = 0; sp < 2; ++sp) {
+    Particles64 P = p64(ctx, sp, ctx->cur[sp]);
+    k_diag_particles<<<kDiagNBP, kBlock, 0, ctx->stream>>>(
+        g, F, P, ctx->n[sp], c.dx, c.c, c.mass[sp], c.weight[sp],
+        sp == PIC_ION ? c.q : -c.q, pp[sp]);
+    PIC_LAUNCH_CHECK(ctx);
+    if (counts && ctx->n[sp] > 0) {
+      k_count_live<<<(unsigned)(div_up(ctx->n[sp], kBlock) < 2048
+                                    ? div_up(ctx->n[sp], kBlock) : 2048),
+                     kBlock, 0, ctx->stream>>>(P.z, ctx->n[sp],
+                                               (int64_t *)(counts + sp));
+      PIC_LAUNCH_CHECK(ctx);
+    }
+  }
+  return 0;
+}
+
+int diagnose(pic_ctx *ctx, pic_diag *out) {
+  const int nbf = kDiagNBF, nbp = kDiagNBP;
+  constexpr int NQF = kDiagNQF, NQP = kDiagNQP;
+  const int64_t need = kDiagPartials;
   if (ctx->diag_partial_cap < need) {
     if (ctx->diag_partial) (void)hipFree(ctx->diag_partial);
     ctx->diag_partial = nullptr;
@@ -1161,16 +1250,8 @@ int diagnose(pic_ctx *ctx, pic_diag *out) {
   }
   double *pf = ctx->diag_partial;
   double *pp[2] = {pf + (int64_t)nbf * NQF, pf + (int64_t)nbf * NQF + (int64_t)nbp * NQP};
-  k_diag_fields<<<nbf, kBlock, 0, ctx->stream>>>(g, F, c.dx, c.eps0, c.mu0,
-                                                 c.q, pf);
-  PIC_LAUNCH_CHECK(ctx);
-  for (int sp = 0; sp < 2; ++sp) {
-    Particles64 P = p64(ctx, sp, ctx->cur[sp]);
-    k_diag_particles<<<nbp, kBlock, 0, ctx->stream>>>(
-        g, F, P, ctx->n[sp], c.dx, c.c, c.mass[sp], c.weight[sp],
-        sp == PIC_ION ? c.q : -c.q, pp[sp]);
-    PIC_LAUNCH_CHECK(ctx);
-  }
+  int rc0 = diagnose_partials(ctx, pf, nullptr);
+  if (rc0) return rc0;
   double *h = (double *)malloc(need * sizeof(double));
   if (!h) return set_error(ctx, PIC_E_NOMEM, "diagnose: host buffer");
   hipError_t e = hipMemcpyAsync(h, pf, need * sizeof(double),

@@ -105,11 +105,27 @@ struct pic_ctx {
   double *diag_partial = nullptr;  // reduction partials
   int64_t diag_partial_cap = 0;
 
+  // thrust_fave accumulated by pic_step (pic_thrust_ave_enable).  The force
+  // of step c is summed during the particle pass of step c + 1, which
+  // gathers the same positions, velocities and E; B of the end of step c is
+  // kept in thrust_bprev before the first B half-step of step c + 1.
+  int thrust_on = 0;
+  int thrust_pending = 0;            // the last step's force not yet added
+  int thrust_now = 0;                // the running pass adds it
+  int64_t thrust_steps = 0;          // steps added to thrust_acc
+  double *thrust_acc = nullptr;      // device: per species E part (3), v x B part (3)
+  double *thrust_part = nullptr;     // device: 6 partials per pass workgroup
+  int64_t thrust_part_slots = 0;
+  void *thrust_bprev = nullptr;      // device: 3 FP64 B planes (the FP32 form
+                                     // keeps its copy beside its total-B planes)
+
   // sort tiles: per-species particle ranges of each tile after the last
   // sort; [0, sorted_end) is tile-sorted, later slots are the inflow tail
   uint32_t *tile_start[2] = {nullptr, nullptr};   // ntiles + 1 per species
   int64_t sorted_end[2] = {0, 0};
   int32_t drift_halves[2] = {0, 0};  // half-step drifts since the tile sort
+  int64_t sort_tile_count = 0;   // sorts done by the tile sort (FP32 form)
+  int64_t sort_radix_count = 0;  // sorts done by the radix sort
 
   // state private to the numeric form (FP32: sigma copy, tile ranges,
   // deterministic accumulators)
@@ -246,6 +262,29 @@ int refresh_total_b(pic_ctx *);
 int download_raw32(pic_ctx *, int sp, int64_t cap, int64_t *n, int32_t *cell,
                    float *const *f);
 }  // namespace f32
+
+// Diagnostics partials, the same layout in both forms: kDiagNBF workgroup
+// partials of kDiagNQF field quantities, then per species kDiagNBP
+// workgroup partials of kDiagNQP particle quantities.  Summed in workgroup
+// order per quantity (quantity 5 of the field partials by max).
+constexpr int kDiagNBF = 1024, kDiagNBP = 2048;
+constexpr int kDiagNQF = 9, kDiagNQP = 21;
+constexpr int64_t kDiagPartials =
+    (int64_t)kDiagNBF * kDiagNQF + 2 * (int64_t)kDiagNBP * kDiagNQP;
+namespace f64 {
+// launch the partial sums into partials[kDiagPartials] and add the live
+// particle counts of the two species into counts[0..1] (zeroed by the caller)
+int diagnose_partials(pic_ctx *, double *partials, unsigned long long *counts);
+}
+namespace f32 {
+int diagnose_partials(pic_ctx *, double *partials, unsigned long long *counts);
+}
+// thrust_fave: add one pass's workgroup partials (6 per workgroup, nb
+// workgroups) times qn = q_s w dx^3 to the species' running sums
+int thrust_add(pic_ctx *ctx, int sp, int nb, double qn);
+namespace f32 {
+float *total_b(pic_ctx *);  // the 3 total-B planes the gather reads
+}
 
 // the fused particle pass of one species (gather, push, J and n deposit,
 // particle boundary), as pic_step runs it (pic_api.hip)

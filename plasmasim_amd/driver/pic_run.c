@@ -185,6 +185,7 @@ int main(int argc, char **argv) {
   char thrust_path[64], ave_path[64];
   snprintf(thrust_path, sizeof thrust_path, "thrust%d.txt", version);
   snprintf(ave_path, sizeof ave_path, "thrust%d_ave.txt", version);
+  if (fused && thrust_ave && thrust_every > 0) CHECK(ctx, pic_thrust_ave_enable(ctx, 1));
   for (int64_t c = c0; c < steps; ++c) {
     if (fused) {
       CHECK(ctx, pic_step(ctx, 1));
@@ -197,20 +198,29 @@ int main(int argc, char **argv) {
       CHECK(ctx, pic_output_fields(ctx, name, tecplot ? PIC_OUTPUT_TECPLOT
                                                       : PIC_OUTPUT_COLUMNS, c));
     }
-    if (thrust_every > 0 && (c % thrust_every == 0 || thrust_ave)) {
+    /* the fused step sums thrust_fave's force itself (no second gather) */
+    const int ave_fused = fused && thrust_ave;
+    if (thrust_every > 0 && (c % thrust_every == 0 || (thrust_ave && !ave_fused))) {
       pic_diag d;
       double F[27];
       CHECK(ctx, pic_diagnose(ctx, &d));
       forces(&d, F);
       if (c % thrust_every == 0)                        /* thrust_f :917   */
         CHECK(ctx, thrust_line(thrust_path, c, F, d.energy_box, 1));
-      if (thrust_ave) {                                 /* thrust_fave     */
+      if (thrust_ave && !ave_fused) {                   /* thrust_fave     */
         for (int i = 0; i < 27; ++i) Fave[i] += F[i] / thrust_every;
         if (c % thrust_every == 0) {
           CHECK(ctx, thrust_line(ave_path, c, Fave, NULL, 0));
           for (int i = 0; i < 27; ++i) Fave[i] = 0.;
         }
       }
+    }
+    if (thrust_every > 0 && ave_fused && c % thrust_every == 0) {
+      double F[27];
+      int64_t nsum = 0;
+      CHECK(ctx, pic_thrust_ave_get(ctx, F, &nsum, 1)); /* thrust_fave :5265-5281 */
+      for (int i = 0; i < 27; ++i) Fave[i] = F[i] / thrust_every;
+      CHECK(ctx, thrust_line(ave_path, c, Fave, NULL, 0));
     }
     const int last = c + 1 == steps;
     if (suspend_path && ((suspend_every > 0 && (c + 1) % suspend_every == 0) || last))

@@ -66,6 +66,67 @@ def test_tile_sort_permutation_matches_oracle():
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("tile", [(4, 4, 8), (8, 4, 4), (2, 2, 2)])
+def test_tile_sort_after_steps_matches_oracle(tile):
+    """The tile sort meets what a periodic sort meets: particles tile-sorted
+    by the last sort, most still in their tile, some moved to other tiles,
+    dead slots and an inflow tail.  The live particles it leaves are the
+    oracle's stable sort of the same state, bit for bit and in order, and
+    the sort took the tile path (not the radix sort)."""
+    c = cfg(1, tile=tile)
+    c.sort_every, c.inflow_every = 1000, 5
+    g = ps.Simulation(c)
+    g.load_particles()
+    g.begin()
+    g.step(12)
+    pre = [g.download_particles(sp) for sp in (0, 1)]
+    t0, r0 = g.sort_stats()
+    g.sort(0)
+    g.sort(1)
+    t1, r1 = g.sort_stats()
+    assert (t1 - t0, r1 - r0) == (2, 0)
+    o = Oracle(cfg(1, precision=ps.PIC_FP64, tile=tile))
+    for sp in (0, 1):
+        o.upload_particles(sp, pre[sp])
+        o.sort(sp)
+        a, b = g.download_particles(sp), o.download_particles(sp)
+        assert a.shape == b.shape and len(a) > 0
+        assert np.array_equal(a, b), sp
+    # the steps after it run on the new tile ranges
+    g.step(3)
+    assert g.particle_count(0) > 0
+    g.close()
+
+
+def test_tile_sort_falls_back_when_movers_overflow():
+    """20000 electrons in one 2 x 2 x 2 tile of the ballistic zone (drift
+    only) move at 0.9 c along z for three steps after a sort: more than a
+    tile's mover list holds enter the next tile, so the sort falls back to
+    the radix sort, and the result is still the oracle's stable sort."""
+    c = cfg(1, tile=(2, 2, 2))
+    c.sort_every, c.inflow_every = 1000, 1000
+    g = ps.Simulation(c)
+    rng = np.random.default_rng(9)
+    n = 20000
+    p = np.column_stack([rng.uniform(4, 6, n), rng.uniform(4, 6, n), rng.uniform(4, 6, n),
+                         np.zeros(n), np.zeros(n), np.full(n, 0.9)])
+    cell = np.floor(p[:, :3])
+    p[:, :3] = cell + np.minimum(np.float32(p[:, :3] - cell).astype(np.float64), 0.99999994)
+    g.upload_particles(1, p)
+    g.sort(1)  # uploaded, unsorted: radix sort; the tiles are current after it
+    g.step(3)
+    pre = g.download_particles(1)
+    t0, r0 = g.sort_stats()
+    g.sort(1)
+    t1, r1 = g.sort_stats()
+    assert (t1 - t0, r1 - r0) == (0, 1)
+    o = Oracle(cfg(1, precision=ps.PIC_FP64, tile=(2, 2, 2)))
+    o.upload_particles(1, pre)
+    o.sort(1)
+    assert np.array_equal(g.download_particles(1), o.download_particles(1))
+    g.close()
+
+
 def sims_with_state(f, p=None):
     out = []
     for prec in (ps.PIC_FP32, ps.PIC_FP64):
@@ -274,6 +335,35 @@ def test_config5_largest_workload_runs():
     s.close()
 
 
+def test_config5_deterministic_runs_bit_identical():
+    """256^3 x 32 ppc x 2 species (1,073,741,824 particles) in the
+    deterministic sorted-tile deposit mode: two runs of 21 steps (inflows at
+    steps 5, 10, 15 and 20, the tile sort at step 20) end with every field
+    plane bit-identical, the FP64 diagnostics (sums over all particles, so
+    any differing particle bit would show) identical and the counts equal."""
+    import hashlib
+    out = []
+    for run in range(2):
+        c = ps.preset(5)
+        c.deposit = ps.PIC_DEPOSIT_DETERMINISTIC
+        s = ps.Simulation(c)
+        s.load_particles()
+        s.begin()
+        s.step(21)
+        h = {n: hashlib.sha256(s.download_field(n).tobytes()).hexdigest()
+             for n in ("ex", "ey", "ez", "bx", "by", "bz", "jix", "jiy", "jiz", "jex",
+                       "jey", "jez", "ni", "ne")}
+        d = s.diagnose()
+        stats = s.sort_stats()
+        s.close()
+        out.append((h, d, stats))
+    (ha, da, sa), (hb, db, sb) = out
+    assert ha == hb
+    for k in da:
+        assert np.array_equal(np.asarray(da[k]), np.asarray(db[k])), k
+    assert sa == sb and sa[0] >= 2  # both species sorted by the tile sort at step 20
+
+
 def drifted_state(deposit, steps=8, seed=17):
     """FP32 state in which a few percent of particles have left the LDS window
     of the tile they were sorted into: a thermal population (0.05 c) with 2 %
@@ -397,3 +487,46 @@ def test_inflow_into_dead_slots_matches_fp64_as_set(deposit):
             assert np.abs(np.sort(a[:, j]) - np.sort(b[:, j])).max() <= tol, (sp, j)
         for x, y in zip(raws[0][sp], raws[1][sp]):
             assert np.array_equal(x, y), sp
+
+
+@pytest.mark.parametrize("tile", [(4, 4, 0), (4, 4, 32), (8, 4, 4), (8, 4, 16)])
+def test_inflow_set_for_other_tile_shapes(tile):
+    """The inflow batch after a sort is the FP64 form's as a set for tilings
+    with one z tile layer (both faces' slabs then lie in the same tiles: the
+    batch is appended, not written into dead slots by two workgroups at
+    once) and for the 8 x 4 x 4 tiles of the 128^3 workload (tiles longer in
+    x than in y), with both faces refilled.  Counts per cell are exact,
+    positions agree to fp32 rounding of the offsets."""
+    out = {}
+    for prec in (ps.PIC_FP32, ps.PIC_FP64):
+        c = cfg(1, precision=prec, tile=tile)
+        c.refill_far = 1
+        s = ps.Simulation(c)
+        s.load_particles()
+        # a few steps first, so that particles left the slabs and the
+        # boundary marked dead slots, then the sort and two inflow events
+        s.begin()
+        s.step(3)
+        s.sort(0)
+        s.sort(1)
+        s.inflow()
+        s.inflow()
+        out[prec] = [s.download_particles(sp) for sp in (0, 1)]
+        s.close()
+    for sp in (0, 1):
+        a, b = out[ps.PIC_FP32][sp], out[ps.PIC_FP64][sp]
+        assert a.shape == b.shape, (tile, sp)
+        # the batch particles (slab layers z in [2, 4) and [NZ, NZ + 2))
+        # carry no push and match per cell exactly
+        for x in (a, b):
+            assert np.isfinite(x).all()
+        za, zb = a[:, 2], b[:, 2]
+        sa = (za < 4) | (za >= 32)
+        sb = (zb < 4) | (zb >= 32)
+        assert sa.sum() == sb.sum(), (tile, sp)
+        ca, cb = (np.unique(np.floor(x[m, :3]), axis=0, return_counts=True)
+                  for x, m in ((a, sa), (b, sb)))
+        assert np.array_equal(ca[0], cb[0]) and np.array_equal(ca[1], cb[1]), (tile, sp)
+        for j in range(6):
+            tol = 2.4e-7 if j < 3 else 6e-8 * np.abs(b[sb, j]).max()
+            assert np.abs(np.sort(a[sa, j]) - np.sort(b[sb, j])).max() <= tol, (tile, sp, j)
