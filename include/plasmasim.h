@@ -217,6 +217,16 @@ int pic_inflow(pic_ctx *ctx);
 int pic_sort(pic_ctx *ctx, int species);
 /* Diagnostics (thrust_f :4940-5122 + Gauss residual :8345-8348).  Syncs. */
 int pic_diagnose(pic_ctx *ctx, pic_diag *out);
+/* No counterpart in the reference: pic_diagnose for callers that cannot
+ * allocate (code under HIP graph capture, loops that forbid hipMalloc).
+ * Writes the same record to dev_out, device memory of pic_sizeof_diag()
+ * bytes owned by the caller, asynchronously on the context stream; calls no
+ * allocator and does not synchronise.  Its workgroup partials and counters
+ * live in a static device array of 95,234 doubles (744 KiB) per GPU, held
+ * until the process ends; the layout is fixed, so every context fits.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (several
+ * contexts' streams at once) race. */
+int pic_diagnose_device(pic_ctx *ctx, pic_diag *dev_out);
 
 /* fdtd() step loop (:687-949): nsteps full steps with the configured sort
  * and inflow cadences, fusing loops A and B and the particle boundary into

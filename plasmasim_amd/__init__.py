@@ -193,6 +193,12 @@ class Simulation:
                                                1 if reset else 0))
         return F, n.value
 
+    def diagnose_device(self, dev_ptr):
+        """pic_diagnose_device: the diagnostics record into device memory at
+        dev_ptr (an integer address of pic_sizeof_diag() bytes), queued on
+        the context stream without synchronising."""
+        self._check(self._L.pic_diagnose_device(self._ctx, ctypes.c_void_p(int(dev_ptr))))
+
     def sync(self):
         self._check(self._L.pic_sync(self._ctx))
 

@@ -162,6 +162,7 @@ SIGNATURES = [
     ("pic_inflow", ctypes.c_int, [_P]),
     ("pic_sort", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_diagnose", ctypes.c_int, [_P, ctypes.POINTER(PicDiag)]),
+    ("pic_diagnose_device", ctypes.c_int, [_P, _P]),
     ("pic_step", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_thrust_ave_enable", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic_thrust_ave_get", ctypes.c_int, [_P, _D, _I64P, ctypes.c_int]),

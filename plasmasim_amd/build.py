@@ -36,6 +36,7 @@ UNITS = {
     "pic_ckpt.hip": ["-ffp-contract=off"],
     "pic_output.cpp": ["-ffp-contract=off"],
     "pic_poisson.hip": ["-ffp-contract=off"],
+    "pic_noalloc.hip": ["-ffp-contract=off"],
 }
 
 

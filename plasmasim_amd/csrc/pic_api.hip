@@ -373,6 +373,12 @@ extern "C" int pic_create(const pic_config *cfg, pic_ctx **out) {
   } while (0)
 
   CREATE_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  // this device's static scratch for the allocation-free entry points
+  if ((rc = noalloc_init(ctx))) {
+    free_all(ctx);
+    delete ctx;
+    return rc;
+  }
   ctx->field_bytes = (size_t)PIC_NFIELDS * g.plane * esz;
   CREATE_HIP(hipMalloc(&ctx->field_mem, ctx->field_bytes));
   CREATE_HIP(hipMemsetAsync(ctx->field_mem, 0, ctx->field_bytes, ctx->stream));

@@ -104,6 +104,8 @@ struct pic_ctx {
   int64_t *host_counter = nullptr; // pinned mirror
   double *diag_partial = nullptr;  // reduction partials
   int64_t diag_partial_cap = 0;
+  int device = 0;                   // HIP device current at pic_create
+  double *noalloc_scratch = nullptr;  // this device's static scratch array
 
   // thrust_fave accumulated by pic_step (pic_thrust_ave_enable).  The force
   // of step c is summed during the particle pass of step c + 1, which
@@ -285,6 +287,9 @@ int thrust_add(pic_ctx *ctx, int sp, int nb, double qn);
 namespace f32 {
 float *total_b(pic_ctx *);  // the 3 total-B planes the gather reads
 }
+// static device scratch of the allocation-free entry points (pic_noalloc.hip):
+// resolved for the current device at pic_create
+int noalloc_init(pic_ctx *ctx);
 
 // the fused particle pass of one species (gather, push, J and n deposit,
 // particle boundary), as pic_step runs it (pic_api.hip)
