@@ -37,6 +37,8 @@ constexpr double kPi = 3.14159265358979323846;
 
 #define PIC_GLB __attribute__((address_space(1)))
 
+#define PIC_LDS __attribute__((address_space(3)))
+
 __device__ inline void gadd(double *p, double v) {
   __hip_atomic_fetch_add((PIC_GLB double *)p, v, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
@@ -468,18 +470,63 @@ __global__ void k_add_gather_T(int64_t n, Lam L) {
   }
 }
 
+// LDS window of the transposed gather: six FP64 planes of kAdjWin nodes
+// (36 KB) around the workgroup's particles
+constexpr int kAdjWin = 768;
+
 __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
                                                 Lam L, int sp, PassK K,
                                                 double *lp, double *gV_out) {
+  __shared__ double win[6 * kAdjWin];
+  __shared__ int bb[6];  // node box of the workgroup's gathers: lo k, l, m, hi k, l, m
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double gV = 0;
-  if (i < n && !isnan(P.z[i])) {
-    const double x0[3] = {P.x[i], P.y[i], P.z[i]};
-    const double v0[3] = {P.vx[i], P.vy[i], P.vz[i]};
-    const double dx = K.dx, dth = K.dth;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const double dx = K.dx, dth = K.dth;
+  bool live = i < n && !isnan(P.z[i]);
+  double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0};
+  bool pushed = false;
+  if (live) {
+    x0[0] = P.x[i];
+    x0[1] = P.y[i];
+    x0[2] = P.z[i];
+    v0[0] = P.vx[i];
+    v0[1] = P.vy[i];
+    v0[2] = P.vz[i];
     const bool ball = x0[0] < K.blo[0] || x0[0] > K.bhi[0] || x0[1] < K.blo[1] ||
                       x0[1] > K.bhi[1] || x0[2] < K.blo[2] || x0[2] > K.bhi[2];
-    const bool pushed = !ball && in_array(g, x0, dx);
+    pushed = !ball && in_array(g, x0, dx);
+  }
+  // the nodes the gathers at x0 touch: from the half-staggered base (int)(x -
+  // 1/2) to the integer base (int)(x) + 1 on each axis
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = pushed ? (int)(x0[a] / dx - 0.5) : (1 << 29);
+    hi[a] = pushed ? (int)(x0[a] / dx) + 1 : -(1 << 29);
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = min(lo[a], __shfl_xor(lo[a], o));
+      hi[a] = max(hi[a], __shfl_xor(hi[a], o));
+    }
+  if (tid < 3) {
+    bb[tid] = 1 << 29;
+    bb[3 + tid] = -(1 << 29);
+  }
+  __syncthreads();
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bb[a], lo[a]);
+      atomicMax(&bb[3 + a], hi[a]);
+    }
+  __syncthreads();
+  const int k0 = bb[0], l0 = bb[1], m0 = bb[2];
+  const int WX = bb[3] - k0 + 1, WY = bb[4] - l0 + 1, WZ = bb[5] - m0 + 1;
+  const bool on = bb[3] >= k0 && (int64_t)WX * WY * WZ <= kAdjWin;
+  const int WN = on ? WX * WY * WZ : 0;
+  for (int j = tid; j < 6 * WN; j += blockDim.x) win[(j / WN) * kAdjWin + j % WN] = 0.0;
+  __syncthreads();
+  double gV = 0;
+  if (live) {
     double f[6] = {0, 0, 0, 0, 0, 0}, vp[3], x1[3];
     if (pushed) {
       for (int c = 0; c < 6; ++c) {
@@ -522,9 +569,23 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
         const double w = lf[c] * K.inv3;
         for (int a = 0; a < 3; ++a) lx[a] += w * gr[a];
         // the total B = B0 + B1 is gathered: both adjoints receive the same
-        // scatter, collected once in the scratch planes S and added to each
-        // after the pass (k_add_gather_T)
-        st_scatter(g, c < 3 ? L.E[c] : L.S[c - 3], s, w);
+        // scatter, collected once in the scratch planes S (added to each
+        // after the pass, k_add_gather_T); it goes to the workgroup's LDS
+        // window first, each lane walking the 8 nodes in its own order so
+        // that lanes of one cell add to different nodes at a time
+        if (on) {
+          const int bi = s.b[0] - k0, bj = s.b[1] - l0, bk = s.b[2] - m0;
+          for (int t = 0; t < 8; ++t) {
+            const int nn = t ^ (lane & 7);
+            const int a = nn >> 2, b2 = (nn >> 1) & 1, c2 = nn & 1;
+            const int j = ((bi + a) * WY + (bj + b2)) * WZ + (bk + c2);
+            __hip_atomic_fetch_add((PIC_LDS double *)&win[c * kAdjWin + j],
+                                   w * s.w[0][a] * s.w[1][b2] * s.w[2][c2],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else {
+          st_scatter(g, c < 3 ? L.E[c] : L.S[c - 3], s, w);
+        }
       }
     } else {
       for (int a = 0; a < 3; ++a) lv[a] = lvp[a];
@@ -534,12 +595,24 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
       lpi[3 + a] = lv[a];
     }
   }
+  if (on) {
+    __syncthreads();
+    const int WYZ = WY * WZ;
+    for (int c = 0; c < 6; ++c) {
+      double *dst = c < 3 ? L.E[c] : L.S[c - 3];
+      for (int j = tid; j < WN; j += blockDim.x) {
+        const double v = win[c * kAdjWin + j];
+        if (v == 0.0) continue;
+        gadd(dst + g.idx(k0 + j / WYZ, l0 + (j / WZ) % WY, m0 + j % WZ), v);
+      }
+    }
+  }
   // per-block sum of the motional-field V derivative
   __shared__ double red[256];
   red[threadIdx.x] = gV;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+  for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
     __syncthreads();
   }
   if (threadIdx.x == 0 && red[0] != 0.0) gadd(gV_out, red[0]);

@@ -40,12 +40,16 @@ namespace f32 {
 #ifndef PIC_PPT32
 #define PIC_PPT32 4  // particles per thread and vector width of the pass
 #endif
+#ifndef PIC_PASS_BLOCK
+#define PIC_PASS_BLOCK 256  // threads per workgroup of the particle pass
+#endif
 #if PIC_MINWAVES32 > 1
 #define PIC_WAVES32 __attribute__((amdgpu_waves_per_eu(PIC_MINWAVES32, 8)))
 #else
 #define PIC_WAVES32
 #endif
 constexpr int kWin = PIC_KWIN32;  // LDS window capacity in nodes (4 x int64 each)
+constexpr int kPassBlock = PIC_PASS_BLOCK;
 constexpr int kPackShift = 10;
 constexpr int kPackMask = 1023;
 
@@ -573,7 +577,7 @@ __device__ __forceinline__ void particle_step(
 // thrust_fave partials of one workgroup: the six per-lane fp32 sums added
 // in FP64 over the workgroup in a fixed order, stored at slot bid
 __device__ inline void thrust_store(const float (&th)[6], double *out, int bid) {
-  __shared__ double sh[6][kBlock / 64];
+  __shared__ double sh[6][kPassBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
@@ -584,13 +588,13 @@ __device__ inline void thrust_store(const float (&th)[6], double *out, int bid) 
   __syncthreads();
   if (threadIdx.x < 6) {
     double a = sh[threadIdx.x][0];
-    for (int j = 1; j < kBlock / 64; ++j) a += sh[threadIdx.x][j];
+    for (int j = 1; j < kPassBlock / 64; ++j) a += sh[threadIdx.x][j];
     out[(int64_t)bid * 6 + threadIdx.x] = a;
   }
 }
 
 template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, bool THRUST>
-__global__ __launch_bounds__(256) PIC_WAVES32 void k_particles32(
+__global__ __launch_bounds__(kPassBlock) PIC_WAVES32 void k_particles32(
     Grid32 G, TileGeom tg, Particles32 P, TileRun tr, Push32 kp,
     typename GlobT<DET>::T *jx, typename GlobT<DET>::T *jy,
     typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd, double *thrust_part) {
@@ -2049,11 +2053,11 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
                        *jz = base + (off + 2) * pl,
                        *nd = base + (6 + (sp == PIC_ION ? 0 : 1)) * pl;
 #define L(A, B, C) \
-  k_particles32<A, B, C, true, false><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
+  k_particles32<A, B, C, true, false><<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
     switch (sel == 7 ? 8 : sel) {
       case 8:
         k_particles32<true, true, true, true, true>
-            <<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
+            <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
         break;
       case 6: L(true, true, false); break;
       case 4: L(true, false, false); break;
@@ -2072,11 +2076,11 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
     float *jz = F.f[sp == PIC_ION ? PIC_JIZ : PIC_JEZ];
     float *nd = F.f[sp == PIC_ION ? PIC_NI : PIC_NE];
 #define L(A, B, C) \
-  k_particles32<A, B, C, false, false><<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
+  k_particles32<A, B, C, false, false><<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
     switch (sel == 7 ? 8 : sel) {
       case 8:
         k_particles32<true, true, true, false, true>
-            <<<nb, kBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
+            <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
         break;
       case 6: L(true, true, false); break;
       case 4: L(true, false, false); break;
