@@ -27,7 +27,9 @@ COMMON = [
 UNITS = {
     "pic_api.hip": ["-ffp-contract=off"],
     "pic_fp64.hip": ["-ffp-contract=off"],
-    "pic_fp32.hip": ["-ffp-contract=fast"],
+    # the particle pass schedules better with memory clauses kept together
+    # (1.2 % faster pass at 128^3 x 64 ppc)
+    "pic_fp32.hip": ["-ffp-contract=fast", "-mllvm", "-amdgpu-sched-strategy=max-memory-clause"],
     "pic_prim.hip": ["-ffp-contract=off"],
     "pic_adj.hip": ["-ffp-contract=off"],
     "pic2d_api.hip": ["-ffp-contract=off"],

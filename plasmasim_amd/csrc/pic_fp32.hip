@@ -1332,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
   for (int64_t i = q0 + lane; i < q1; i += 64) {
     const uint32_t key = tsort_key(g, tg, cell[i], drop);
     if (key == drop) continue;
-    const uint32_t t = key / cpt;
+    const uint32_t t = tg.tile_of(key);
     if (bid < tr.ntiles && t == (uint32_t)bid) {
       ++kept;
       continue;
@@ -1375,7 +1375,7 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
     const int64_t i = i0 + lane;
     uint32_t key = drop, t = 0;
     if (i < q1) key = tsort_key(g, tg, cell[i], drop);
-    if (key != drop) t = key / cpt;
+    if (key != drop) t = tg.tile_of(key);
     int h = -1;
     if (key != drop && !(bid < tr.ntiles && t == (uint32_t)bid)) h = slot_of(t, false);
     const bool mv = h >= 0;
@@ -1384,7 +1384,7 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
     if (mv) {
       const uint32_t pos = wc[wv][h] + (uint32_t)r;
       if (pos < (uint32_t)kMoverCap)
-        movers[(int64_t)t * kMoverCap + pos] = make_uint2((uint32_t)i, key % cpt);
+        movers[(int64_t)t * kMoverCap + pos] = make_uint2((uint32_t)i, tg.cell_of(key));
     }
     // the lanes' reads above precede this update in the wave's LDS order
     if (mv && r == 0) wc[wv][h] += (uint32_t)cnt;
@@ -1497,8 +1497,8 @@ __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
     }
     slot = seg_src[lo] + r;
     const uint32_t key = tsort_key(g, tg, cell[slot], drop);
-    if (key == drop || key / cpt != (uint32_t)t) return false;
-    c = key % cpt;
+    if (key == drop || tg.tile_of(key) != (uint32_t)t) return false;
+    c = tg.cell_of(key);
     return true;
   };
   const uint32_t per = (total + 3) / 4;

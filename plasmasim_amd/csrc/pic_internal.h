@@ -28,10 +28,25 @@ struct TileGeom {
   int TX, TY, TZ;     // cells per tile
   int NTX, NTY, NTZ;  // tiles per axis
   int shift[3];       // cells by which the tiles' LDS windows are moved
+  int LX, LY, LZ;     // log2 of TX, TY, TZ when all three are powers of two, else -1
   __host__ __device__ inline uint32_t key(int k, int l, int m) const {
+    if (LX >= 0) {  // shifts and masks instead of integer divisions
+      const uint32_t tile =
+          ((uint32_t)(k >> LX) * NTY + (uint32_t)(l >> LY)) * NTZ + (uint32_t)(m >> LZ);
+      const uint32_t in = ((((uint32_t)k & (TX - 1)) << LY | ((uint32_t)l & (TY - 1)))
+                           << LZ) | ((uint32_t)m & (TZ - 1));
+      return (tile << (LX + LY + LZ)) | in;
+    }
     const uint32_t tile = ((uint32_t)(k / TX) * NTY + l / TY) * NTZ + m / TZ;
     const uint32_t in = ((uint32_t)(k % TX) * TY + l % TY) * TZ + m % TZ;
     return tile * (uint32_t)(TX * TY * TZ) + in;
+  }
+  // tile and cell-in-tile of a key
+  __host__ __device__ inline uint32_t tile_of(uint32_t key) const {
+    return LX >= 0 ? key >> (LX + LY + LZ) : key / (uint32_t)(TX * TY * TZ);
+  }
+  __host__ __device__ inline uint32_t cell_of(uint32_t key) const {
+    return LX >= 0 ? key & (uint32_t)(TX * TY * TZ - 1) : key % (uint32_t)(TX * TY * TZ);
   }
   __host__ __device__ inline int ntiles() const { return NTX * NTY * NTZ; }
 };
@@ -215,6 +230,15 @@ inline TileGeom tile_geom(const pic_ctx *ctx) {
   t.NTY = c.ny / t.TY;
   t.NTZ = c.nz / t.TZ;
   for (int a = 0; a < 3; ++a) t.shift[a] = 0;
+  const auto lg = [](int v) {
+    int b = 0;
+    while ((1 << b) < v) ++b;
+    return (1 << b) == v ? b : -1;
+  };
+  t.LX = lg(t.TX);
+  t.LY = lg(t.TY);
+  t.LZ = lg(t.TZ);
+  if (t.LX < 0 || t.LY < 0 || t.LZ < 0) t.LX = t.LY = t.LZ = -1;
   return t;
 }
 
