@@ -1254,16 +1254,34 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 // tile, a few to some tens per slab tile, so a short block meets few tiles
 constexpr int kSortTail = 256;
 
-__device__ inline uint32_t tsort_key(const Geom &g, const TileGeom &tg, int32_t c,
-                                     uint32_t drop) {
-  if (c < 0) return drop;
+// The destination of a stored particle: false when it dropped out, else its
+// key's tile t and cell in the tile cin (key = t * cells per tile + cin) and
+// h, its tile's home slot in a 64-entry table: the tile coordinates modulo
+// 4, so that the 27 tiles around any tile take distinct slots.
+__device__ inline bool tsort_dest(const Geom &g, const TileGeom &tg, int32_t c, uint32_t &t,
+                                  uint32_t &cin, int &h) {
+  if (c < 0) return false;
   int k, l, m;
   unpack(c, k, l, m);
   k -= 2;
   l -= 2;
   m -= 2;
-  if (k < 0 || k >= g.NX || l < 0 || l >= g.NY || m < 0 || m >= g.NZ) return drop;
-  return tg.key(k, l, m);
+  if (k < 0 || k >= g.NX || l < 0 || l >= g.NY || m < 0 || m >= g.NZ) return false;
+  int tx, ty, tz;
+  if (tg.LX >= 0) {
+    tx = k >> tg.LX;
+    ty = l >> tg.LY;
+    tz = m >> tg.LZ;
+  } else {
+    tx = k / tg.TX;
+    ty = l / tg.TY;
+    tz = m / tg.TZ;
+  }
+  const int ix = k - tx * tg.TX, iy = l - ty * tg.TY, iz = m - tz * tg.TZ;
+  t = ((uint32_t)tx * tg.NTY + (uint32_t)ty) * tg.NTZ + (uint32_t)tz;
+  cin = ((uint32_t)ix * tg.TY + (uint32_t)iy) * tg.TZ + (uint32_t)iz;
+  h = ((tx & 3) << 4) | ((ty & 3) << 2) | (tz & 3);
+  return true;
 }
 
 // rank of this lane among the active lanes (pred) of its wave holding the
@@ -1295,27 +1313,27 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
   __shared__ uint32_t wc[kBlock / 64][kDestMax];
   __shared__ unsigned int kept_w[kBlock / 64];
   const int bid = blockIdx.x, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  int64_t s, e;
-  if (bid < tr.ntiles) {
+  const bool own_tile = bid < tr.ntiles;
+  // slots fit 32 bits: the library keeps at most 2^30 particles per species
+  uint32_t s, e;
+  if (own_tile) {
     s = tr.tile_start[bid];
     e = tr.tile_start[bid + 1];
   } else {
-    s = tr.sorted_end + (int64_t)(bid - tr.ntiles) * kSortTail;
-    e = s + kSortTail < tr.n ? s + kSortTail : tr.n;
+    s = (uint32_t)tr.sorted_end + (uint32_t)(bid - tr.ntiles) * kSortTail;
+    e = min(s + (uint32_t)kSortTail, (uint32_t)tr.n);
   }
-  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
-  const uint32_t drop = (uint32_t)tg.ntiles() * cpt;
   for (int h = tid; h < kDestMax; h += kBlock) {
     tkey[h] = kEmpty;
     for (int w = 0; w < kBlock / 64; ++w) wc[w][h] = 0u;
   }
   __syncthreads();
   // this wave's contiguous quarter of the block
-  const int64_t len = e - s, per = (len + 3) / 4;
-  const int64_t q0 = s + wv * per, q1 = min(q0 + per, e);
-  // the destination's table slot (linear probing), -1 when the table is full
-  const auto slot_of = [&](uint32_t t, bool insert) -> int {
-    int h = (int)(t % kDestMax);
+  const uint32_t per = (e - s + 3) / 4;
+  const uint32_t q0 = min(s + wv * per, e), q1 = min(q0 + per, e);
+  // the destination's table slot by linear probing from its home slot h,
+  // -1 when the table is full (insert) or the tile is absent (find)
+  const auto slot_of = [&](uint32_t t, int h, bool insert) -> int {
     for (int i = 0; i < kDestMax; ++i) {
       const uint32_t cur = tkey[h];
       if (cur == t) return h;
@@ -1324,25 +1342,37 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
         const uint32_t old = atomicCAS(&tkey[h], kEmpty, t);
         if (old == kEmpty || old == t) return h;
       }
-      h = (h + 1) % kDestMax;
+      h = (h + 1) & (kDestMax - 1);
     }
     return -1;
   };
+  // a particle's table slot when it moves to another tile, else -1
+  const auto classify = [&](int32_t c, bool insert, uint32_t &t, uint32_t &cin,
+                            bool &kept) -> int {
+    int h0;
+    kept = false;
+    if (!tsort_dest(g, tg, c, t, cin, h0)) return -1;
+    if (own_tile && t == (uint32_t)bid) {
+      kept = true;
+      return -1;
+    }
+    const int h = slot_of(t, h0, insert);
+    if (h < 0 && insert) atomicOr(flag, 1u);
+    return h;
+  };
   unsigned int kept = 0;
-  for (int64_t i = q0 + lane; i < q1; i += 64) {
-    const uint32_t key = tsort_key(g, tg, cell[i], drop);
-    if (key == drop) continue;
-    const uint32_t t = tg.tile_of(key);
-    if (bid < tr.ntiles && t == (uint32_t)bid) {
-      ++kept;
-      continue;
-    }
-    const int h = slot_of(t, true);
-    if (h < 0) {
-      atomicOr(flag, 1u);
-      continue;
-    }
-    atomicAdd(&wc[wv][h], 1u);
+  int32_t cnext = q0 + lane < q1 ? cell[q0 + lane] : -1;
+  for (uint32_t i0 = q0; i0 < q1; i0 += 64) {
+    const int32_t c = cnext;
+    cnext = i0 + 64 + lane < q1 ? cell[i0 + 64 + lane] : -1;
+    uint32_t t = 0, cin = 0;
+    bool kp;
+    const int h = classify(c, true, t, cin, kp);
+    kept += kp;
+    // one LDS add per destination present in the wave
+    int cnt;
+    const int r = wave_rank(h >= 0, (uint32_t)(h >= 0 ? h : 0), 6, cnt);
+    if (h >= 0 && r == 0) atomicAdd(&wc[wv][h], (uint32_t)cnt);
   }
   for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
   if (lane == 0) kept_w[wv] = kept;
@@ -1355,7 +1385,7 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
     const uint32_t off = atomicAdd(&mcount[t], tot);
     const uint32_t k = atomicAdd(&ndesc[t], 1u);
     if (k < (uint32_t)kDescCap)
-      desc[(int64_t)t * kDescCap + k] = make_uint4((uint32_t)s, off, tot, 0u);
+      desc[(int64_t)t * kDescCap + k] = make_uint4(s, off, tot, 0u);
     if (k >= (uint32_t)kDescCap || off + tot > (uint32_t)kMoverCap) atomicOr(flag, 1u);
     uint32_t run = off;
     for (int w = 0; w < kBlock / 64; ++w) {
@@ -1364,27 +1394,27 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
       run += c;
     }
   }
-  if (tid == 0 && bid < tr.ntiles) {
+  if (tid == 0 && own_tile) {
     unsigned int k = 0;
     for (int w = 0; w < kBlock / 64; ++w) k += kept_w[w];
     stay[bid] = k;
   }
   __syncthreads();
   // each wave writes its movers in slot order
-  for (int64_t i0 = q0; i0 < q1; i0 += 64) {
-    const int64_t i = i0 + lane;
-    uint32_t key = drop, t = 0;
-    if (i < q1) key = tsort_key(g, tg, cell[i], drop);
-    if (key != drop) t = tg.tile_of(key);
-    int h = -1;
-    if (key != drop && !(bid < tr.ntiles && t == (uint32_t)bid)) h = slot_of(t, false);
+  cnext = q0 + lane < q1 ? cell[q0 + lane] : -1;
+  for (uint32_t i0 = q0; i0 < q1; i0 += 64) {
+    const int32_t c = cnext;
+    cnext = i0 + 64 + lane < q1 ? cell[i0 + 64 + lane] : -1;
+    uint32_t t = 0, cin = 0;
+    bool kp;
+    const int h = classify(c, false, t, cin, kp);
     const bool mv = h >= 0;
     int cnt;
     const int r = wave_rank(mv, (uint32_t)(mv ? h : 0), 6, cnt);
     if (mv) {
       const uint32_t pos = wc[wv][h] + (uint32_t)r;
       if (pos < (uint32_t)kMoverCap)
-        movers[(int64_t)t * kMoverCap + pos] = make_uint2((uint32_t)i, tg.cell_of(key));
+        movers[(int64_t)t * kMoverCap + pos] = make_uint2(i0 + (uint32_t)lane, cin);
     }
     // the lanes' reads above precede this update in the wave's LDS order
     if (mv && r == 0) wc[wv][h] += (uint32_t)cnt;
@@ -1417,7 +1447,9 @@ __global__ __launch_bounds__(1024) void k_tsort_scan(const uint32_t *stay,
   if (threadIdx.x == 1023) start[nt] = part[1023];
 }
 
-// pass 3: one workgroup per tile t; writes idx[start[t] + position] = slot
+// pass 3: one workgroup per tile t; writes idx[start[t] + position] = slot.
+// Dynamic LDS: per-wave cell counters (4 x cells per tile) and the cells'
+// totals (cells per tile), 20 bytes per cell.
 __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
                                                      const uint32_t *__restrict__ old_start,
                                                      const int32_t *__restrict__ cell,
@@ -1426,93 +1458,112 @@ __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
                                                      const uint2 *__restrict__ movers,
                                                      const uint32_t *__restrict__ start,
                                                      uint32_t *idx) {
-  // the sequence as segments: first sequence index, source offset, kind
-  // (0: movers at that offset of the list, 1: the range from that slot)
+  // the sequence as segments: first sequence index and source offset; the
+  // segment s_own is the tile's own range from slot s0, the others are
+  // mover blocks at that offset of the tile's list
   __shared__ uint32_t seg_q[kDescCap + 2], seg_src[kDescCap + 2];
-  __shared__ int seg_kind[kDescCap + 2];
-  __shared__ int s_nseg;
+  __shared__ int s_own;
   __shared__ uint4 d[kDescCap];
-  __shared__ uint32_t cnt[kBlock / 64][kTileCells];
-  __shared__ uint32_t tot[kTileCells];
   __shared__ uint32_t part[kBlock];
+  extern __shared__ uint32_t place_lds[];
   const int t = blockIdx.x, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
-  const uint32_t drop = (uint32_t)tg.ntiles() * cpt;
+  uint32_t *cnt = place_lds + wv * cpt;  // this wave's counters
+  uint32_t *tot = place_lds + 4 * cpt;
   const uint32_t s0 = old_start[t], len = old_start[t + 1] - s0;
-  if (tid == 0) {
-    // mover blocks by first slot (insertion sort of at most kDescCap), the
-    // own range spliced in at its first slot
-    const int nd = (int)min(ndesc[t], (uint32_t)kDescCap);
-    for (int i = 0; i < nd; ++i) {
-      uint4 x = desc[(int64_t)t * kDescCap + i];
-      int j = i;
-      while (j > 0 && d[j - 1].x > x.x) {
-        d[j] = d[j - 1];
-        --j;
-      }
-      d[j] = x;
-    }
-    uint32_t q = 0;
-    int ns = 0;
-    bool own = false;
-    for (int i = 0; i <= nd; ++i) {
-      if (!own && (i == nd || d[i].x > s0)) {
-        seg_q[ns] = q;
-        seg_src[ns] = s0;
-        seg_kind[ns++] = 1;
-        q += len;
-        own = true;
-      }
-      if (i == nd) break;
-      seg_q[ns] = q;
-      seg_src[ns] = d[i].y;
-      seg_kind[ns++] = 0;
-      // within the list (a block past its end overflowed: the result is
-      // then discarded, but every read stays inside the list)
-      const uint32_t off = min(d[i].y, (uint32_t)kMoverCap);
-      q += min(d[i].z, (uint32_t)kMoverCap - off);
-    }
-    seg_q[ns] = q;  // end of the sequence
-    s_nseg = ns;
+  const int nd = (int)min(ndesc[t], (uint32_t)kDescCap);
+  if (wv == 0) {
+    // the mover blocks in order of first slot: the first slots differ, so
+    // each block's rank is the number of blocks with a smaller one
+    uint4 x = make_uint4(kEmpty, 0u, 0u, 0u);
+    if (lane < nd) x = desc[(int64_t)t * kDescCap + lane];
+    int rank = 0;
+    for (int j = 0; j < nd; ++j) rank += (uint32_t)__builtin_amdgcn_readlane((int)x.x, j) < x.x;
+    if (lane < nd) d[rank] = x;
   }
-  for (int j = tid; j < (int)cpt; j += kBlock)
-    for (int w = 0; w < kBlock / 64; ++w) cnt[w][j] = 0u;
+  for (uint32_t j = tid; j < 4 * cpt; j += kBlock) place_lds[j] = 0u;
   __syncthreads();
-  const int ns = s_nseg;
+  if (wv == 0) {
+    // the sequence: blocks from lower slots, the own range, blocks from
+    // higher slots.  A block past the list's end overflowed: the result is
+    // then discarded, but its clamped length keeps every read in the list.
+    const uint4 y = lane < nd ? d[lane] : make_uint4(kEmpty, 0u, 0u, 0u);
+    const uint32_t off = min(y.y, (uint32_t)kMoverCap);
+    const uint32_t ln = lane < nd ? min(y.z, (uint32_t)kMoverCap - off) : 0u;
+    const int own = __builtin_popcountll(__builtin_amdgcn_ballot_w64(lane < nd && y.x < s0));
+    uint32_t inc = ln;  // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
+    }
+    const uint32_t below = own > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)inc, own - 1) : 0u;
+    const uint32_t sum = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    if (lane < nd) {
+      const int si = lane < own ? lane : lane + 1;
+      seg_q[si] = inc - ln + (lane < own ? 0u : len);
+      seg_src[si] = y.y;
+    }
+    if (lane == 0) {
+      seg_q[own] = below;
+      seg_src[own] = s0;
+      seg_q[nd + 1] = sum + len;  // end of the sequence
+      s_own = own;
+    }
+  }
+  __syncthreads();
+  const int ns = nd + 1, own = s_own;
   const uint32_t total = seg_q[ns];
-  // element q of the sequence: (slot, cell in the tile), false for a range
-  // slot whose particle left the tile or dropped out
-  const auto elem = [&](uint32_t q, uint32_t &slot, uint32_t &c) -> bool {
-    int lo = 0, hi = ns - 1;  // last segment starting at or before q
+  const uint32_t per = (total + 3) / 4;
+  const uint32_t q0 = min(wv * per, total), q1 = min(q0 + per, total);
+  // this lane's segment: the last one starting at or before q, found once
+  // by bisection, then followed as q grows
+  const auto seg_find = [&](uint32_t q) -> int {
+    int lo = 0, hi = ns - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (seg_q[mid] <= q) lo = mid; else hi = mid - 1;
     }
-    const uint32_t r = q - seg_q[lo];
-    if (seg_kind[lo] == 0) {
-      const uint2 m = movers[(int64_t)t * kMoverCap + seg_src[lo] + r];
+    return lo;
+  };
+  const int sg0 = seg_find(q0 + lane);
+  int sg = sg0;
+  // element q of the sequence: (slot, cell in the tile), false for a range
+  // slot whose particle left the tile or dropped out
+  const auto elem = [&](uint32_t q, uint32_t &slot, uint32_t &c) -> bool {
+    while (sg + 1 < ns && seg_q[sg + 1] <= q) ++sg;
+    const uint32_t r = q - seg_q[sg];
+    if (sg != own) {
+      const uint2 m = movers[(int64_t)t * kMoverCap + seg_src[sg] + r];
       slot = m.x;
       c = m.y;
       return true;
     }
-    slot = seg_src[lo] + r;
-    const uint32_t key = tsort_key(g, tg, cell[slot], drop);
-    if (key == drop || tg.tile_of(key) != (uint32_t)t) return false;
-    c = tg.cell_of(key);
-    return true;
+    slot = s0 + r;
+    uint32_t tt;
+    int h;
+    return tsort_dest(g, tg, cell[slot], tt, c, h) && tt == (uint32_t)t;
   };
-  const uint32_t per = (total + 3) / 4;
-  const uint32_t q0 = min(wv * per, total), q1 = min(q0 + per, total);
-  for (uint32_t q = q0 + lane; q < q1; q += 64) {
-    uint32_t slot, c;
-    if (elem(q, slot, c)) atomicAdd(&cnt[wv][c], 1u);
+  for (uint32_t qb = q0; qb < q1; qb += 64) {
+    const uint32_t q = qb + lane;
+    uint32_t slot, c = 0;
+    const bool ok = q < q1 && elem(q, slot, c);
+    // lanes in runs of one cell: the run's first lane adds the run's length
+    const uint32_t cv = ok ? c : kEmpty;
+    const uint32_t cp = __shfl_up(cv, 1);
+    const bool first = lane == 0 || cv != cp;
+    const u64 heads = __builtin_amdgcn_ballot_w64(first);
+    if (ok && first) {
+      const u64 above = (heads >> lane) >> 1;
+      const uint32_t run = above ? (uint32_t)__builtin_ctzll(above) + 1u : 64u - lane;
+      atomicAdd(&cnt[c], run);
+    }
   }
   __syncthreads();
   // cell starts (exclusive scan over the cells' totals), then each wave's
   // start per cell
   for (uint32_t c = tid; c < cpt; c += kBlock) {
     uint32_t v = 0;
-    for (int w = 0; w < kBlock / 64; ++w) v += cnt[w][c];
+    for (int w = 0; w < kBlock / 64; ++w) v += place_lds[w * cpt + c];
     tot[c] = v;
   }
   __syncthreads();
@@ -1531,22 +1582,23 @@ __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
   uint32_t run = start[t] + (tid > 0 ? part[tid - 1] : 0u);
   for (uint32_t c = ca; c < cb; ++c) {
     for (int w = 0; w < kBlock / 64; ++w) {
-      const uint32_t v = cnt[w][c];
-      cnt[w][c] = run;
+      const uint32_t v = place_lds[w * cpt + c];
+      place_lds[w * cpt + c] = run;
       run += v;
     }
   }
   __syncthreads();
   int nbits = 1;
   while ((1u << nbits) < cpt) ++nbits;
+  sg = sg0;
   for (uint32_t qb = q0; qb < q1; qb += 64) {
     const uint32_t q = qb + lane;
     uint32_t slot = 0, c = 0;
     const bool ok = q < q1 && elem(q, slot, c);
     int n;
     const int r = wave_rank(ok, c, nbits, n);
-    if (ok) idx[cnt[wv][c] + (uint32_t)r] = slot;
-    if (ok && r == 0) cnt[wv][c] += (uint32_t)n;
+    if (ok) idx[cnt[c] + (uint32_t)r] = slot;
+    if (ok && r == 0) cnt[c] += (uint32_t)n;
   }
 }
 
@@ -2242,7 +2294,8 @@ static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
       g, tg, tr, src.cell, S->ts_stay, S->ts_mcount, S->ts_ndesc, S->ts_desc,
       S->ts_movers, S->ts_flag);
   k_tsort_scan<<<1, 1024, 0, ctx->stream>>>(S->ts_stay, S->ts_mcount, nt, S->ts_start);
-  k_tsort_place<<<(unsigned)nt, kBlock, 0, ctx->stream>>>(
+  const size_t place_lds = (size_t)5 * tg.TX * tg.TY * tg.TZ * sizeof(uint32_t);
+  k_tsort_place<<<(unsigned)nt, kBlock, place_lds, ctx->stream>>>(
       g, tg, ctx->tile_start[sp], src.cell, S->ts_ndesc, S->ts_desc, S->ts_movers,
       S->ts_start, ctx->idx);
   PIC_LAUNCH_CHECK(ctx);
