@@ -4,17 +4,20 @@
 // drift speed.
 //
 // Forward (pic_adj_forward): the context's own step (either numeric form)
-// with the sort switched off, so every particle keeps its slot (inflow and
-// the particle boundary mark dead slots in place); a checkpoint of the whole
-// state every ckpt_every steps.
+// with the cell sort at its cadence and the inflow appended to the store
+// (each injected batch contiguous; the particle boundary marks dead slots
+// in place); a checkpoint of the whole state every ckpt_every steps.
 //
 // Backward (pic_adj_backward): per segment, from the last, the checkpoint is
 // restored and the segment recomputed while its tape is recorded (particles
-// and fields in FP64 at the start of each step); the steps are then reversed
+// and fields in FP64 at the start of each step, and each sort's
+// permutation); the steps are then reversed
 // in FP64: transposed (2,4) FDTD, guard copies and fold, the loss and the
 // particle pass (transposed gather, relativistic Boris step and current
-// deposit, recomputed from the tape).  All adjoint planes, particle
-// adjoints and the parameter gradients accumulate in FP64.
+// deposit, recomputed from the tape), and across a sort the particle
+// adjoints move back to the slots the particles held before it.  All
+// adjoint planes, particle adjoints and the parameter gradients accumulate
+// in FP64.
 //
 // The forward state is the B1 = B - B0 formulation: B0 = sum_a M_a G_a + IMF
 // enters the gather, the loss and the guard copies of the total B; V enters
@@ -768,6 +771,17 @@ __global__ __launch_bounds__(256) void k_sum_vz(double *lp, int64_t lo, int64_t 
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
+// the particle adjoints across a cell sort, backwards: the particle at slot
+// i after the sort sat at slot perm[i] before it; the slots of particles
+// the sort dropped (dead ones) keep the zero they were given
+__global__ void k_unsort_lp(int64_t na, const uint32_t *__restrict__ perm,
+                            const double *__restrict__ src, double *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const int64_t j = perm[i];
+  for (int a = 0; a < 6; ++a) dst[6 * j + a] = src[6 * i + a];
+}
+
 // out[b * 3 + a] = per-block sums of lB0 . G_a
 __global__ __launch_bounds__(256) void k_dot_g(int64_t n, Lam L, const double *G,
                                                double *out) {
@@ -796,14 +810,19 @@ struct Ckpt {
   uint32_t *tiles[2] = {nullptr, nullptr};
   int64_t n[2], sorted_end[2];
   int cur[2];
+  int32_t drift_halves[2];
   int64_t step;
   int32_t c_add, c_sort;
   uint32_t inflow_event;
 };
 
+// One recorded step.  A step that began with the cell sort keeps, per
+// species, the count before it (nb) and after it (na); its permutation
+// (slot after = perm index, slot before = perm value) goes to the caller's
+// buffers.
 struct TapeStep {
-  int inflow;
-  int64_t n[2], lo[2], hi[2];
+  int inflow, sorted;
+  int64_t n[2], lo[2], hi[2], nb[2], na[2];
 };
 
 struct AdjState {
@@ -969,6 +988,7 @@ int save_ckpt(pic_ctx *ctx, Ckpt &c) {
     c.n[s] = ctx->n[s];
     c.sorted_end[s] = ctx->sorted_end[s];
     c.cur[s] = ctx->cur[s];
+    c.drift_halves[s] = ctx->drift_halves[s];
     const int64_t n = ctx->n[s];
     PIC_HIP(ctx, hipMalloc(&c.parts[s], (size_t)(n + 1) * slot_bytes(ctx)));
     char *src[7], *dst[7];
@@ -1000,6 +1020,7 @@ int restore_ckpt(pic_ctx *ctx, const Ckpt &c) {
     ctx->n[s] = c.n[s];
     ctx->sorted_end[s] = c.sorted_end[s];
     ctx->cur[s] = c.cur[s];
+    ctx->drift_halves[s] = c.drift_halves[s];
     const int64_t n = c.n[s];
     char *src[7], *dst[7];
     size_t esz[7];
@@ -1066,13 +1087,28 @@ int force_now(pic_ctx *ctx, AdjState *A, double *F) {
   return 0;
 }
 
-// one forward step with the sort off; records the inflow ranges and, when
-// tape_fields is given, the tape entry (after the inflow, before the first
-// half step)
-int fwd_step(pic_ctx *ctx, TapeStep &ts, double *tape_fields, PD tape_p[2]) {
+// one forward step: the cell sort at the step loop's cadence (:7475-7590,
+// its permutation into perm[s] when given), the inflow appended to the
+// store (its ranges recorded), and, when tape_fields is given, the tape
+// entry (after the inflow, before the first half step)
+int fwd_step(pic_ctx *ctx, TapeStep &ts, double *tape_fields, PD tape_p[2],
+             uint32_t *const perm[2]) {
   const pic_config &c = ctx->cfg;
   int rc;
   if ((rc = pic_renew_grid(ctx))) return rc;
+  ts.sorted = 0;
+  if (ctx->c_sort > c.sort_every - 1) {
+    ctx->c_sort = 0;
+    for (int s = 0; s < 2; ++s) {
+      ts.nb[s] = ctx->n[s];
+      if ((rc = pic_sort(ctx, s))) return rc;
+      ts.na[s] = ctx->n[s];
+      if (perm && perm[s] && ts.na[s] > 0)
+        PIC_HIP(ctx, hipMemcpyAsync(perm[s], ctx->idx, (size_t)ts.na[s] * sizeof(uint32_t),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ts.sorted = 1;
+  }
   ts.inflow = 0;
   if (ctx->c_add > c.inflow_every - 1) {
     const int nface = c.refill_far ? 2 : 1;
@@ -1199,7 +1235,7 @@ extern "C" int pic_adj_forward(pic_ctx *ctx, const double params[4], int nsteps,
   for (int comp = 0; comp < 3; ++comp)
     if ((rc = pic_upload_field(ctx, PIC_EX + comp, zero.data()))) return rc;
   ctx->cfg.vdrift[2] = params[3];
-  ctx->adj_nosort = 1;
+  ctx->adj_mode = 1;
   ctx->step = 0;
   ctx->c_add = 0;
   ctx->c_sort = 0;
@@ -1213,12 +1249,12 @@ extern "C" int pic_adj_forward(pic_ctx *ctx, const double params[4], int nsteps,
       A->ck.emplace_back();
       if ((rc = save_ckpt(ctx, A->ck.back()))) return rc;
     }
-    if ((rc = fwd_step(ctx, ts, nullptr, nullptr))) return rc;
+    if ((rc = fwd_step(ctx, ts, nullptr, nullptr, nullptr))) return rc;
     double F;
     if ((rc = force_now(ctx, A, &F))) return rc;
     L += F / nsteps;
   }
-  ctx->adj_nosort = 0;
+  ctx->adj_mode = 0;
   A->loss = L;
   A->have_forward = true;
   *loss = L;
@@ -1237,20 +1273,26 @@ extern "C" int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]) {
   int rc;
   hipStream_t st = ctx->stream;
   PIC_HIP(ctx, hipMemsetAsync(A->lam, 0, 18 * n * sizeof(double), st));
-  // particle adjoints and the tape of one segment
-  double *lp[2] = {nullptr, nullptr};
+  // particle adjoints (and their copy across a sort), the tape of one
+  // segment and the permutations of the sorts in it
+  double *lp[2] = {nullptr, nullptr}, *lq[2] = {nullptr, nullptr};
   std::vector<double *> tf(K + 1, nullptr);
   std::vector<double *> tpart(K, nullptr);
+  std::vector<uint32_t *> perm(2 * (size_t)K, nullptr);
   std::vector<TapeStep> ts(K);
   const int64_t cap = ctx->cap;
   auto cleanup = [&]() {
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s) {
       if (lp[s]) (void)hipFree(lp[s]);
+      if (lq[s]) (void)hipFree(lq[s]);
+    }
+    for (uint32_t *p : perm)
+      if (p) (void)hipFree(p);
     for (double *p : tf)
       if (p) (void)hipFree(p);
     for (double *p : tpart)
       if (p) (void)hipFree(p);
-    ctx->adj_nosort = 0;
+    ctx->adj_mode = 0;
   };
 #define TRY(x)          \
   do {                  \
@@ -1263,6 +1305,7 @@ extern "C" int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]) {
   for (int s = 0; s < 2; ++s) {
     TRYH(hipMalloc(&lp[s], 6 * (size_t)cap * sizeof(double)));
     TRYH(hipMemsetAsync(lp[s], 0, 6 * (size_t)cap * sizeof(double), st));
+    TRYH(hipMalloc(&lq[s], 6 * (size_t)cap * sizeof(double)));
   }
   for (int j = 0; j <= K; ++j) TRYH(hipMalloc(&tf[j], 6 * n * sizeof(double)));
   for (int j = 0; j < K; ++j) TRYH(hipMalloc(&tpart[j], 12 * (size_t)cap * sizeof(double)));
@@ -1279,14 +1322,21 @@ extern "C" int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]) {
   double gV = 0;
   double *gvd = A->red + 4000;  // device accumulator of the motional-field term
   TRYH(hipMemsetAsync(gvd, 0, sizeof(double), st));
-  ctx->adj_nosort = 1;
+  ctx->adj_mode = 1;
   const int nseg = (int)A->ck.size();
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int s0 = seg * K, s1 = std::min(N, s0 + K);
     TRY(restore_ckpt(ctx, A->ck[seg]));
     for (int j = 0; j < s1 - s0; ++j) {
       PD tp[2] = {tape_pd(j, 0), tape_pd(j, 1)};
-      TRY(fwd_step(ctx, ts[j], tf[j], tp));
+      uint32_t *pm[2] = {nullptr, nullptr};
+      if (ctx->c_sort > c.sort_every - 1)  // this step begins with the sort
+        for (int s = 0; s < 2; ++s) {
+          uint32_t *&b = perm[2 * (size_t)j + s];
+          if (!b) TRYH(hipMalloc(&b, (size_t)cap * sizeof(uint32_t)));
+          pm[s] = b;
+        }
+      TRY(fwd_step(ctx, ts[j], tf[j], tp, pm));
     }
     TRY(fields_to_d(ctx, tf[s1 - s0]));
     for (int j = s1 - s0 - 1; j >= 0; --j) {
@@ -1366,6 +1416,17 @@ extern "C" int pic_adj_backward(pic_ctx *ctx, double dloss, double grad[4]) {
             return rc;
           }
         }
+      // back across the sort this step began with: the adjoints to the
+      // slots the particles held before it
+      if (ts[j].sorted)
+        for (int sp = 0; sp < 2; ++sp) {
+          TRYH(hipMemsetAsync(lq[sp], 0, 6 * (size_t)ts[j].nb[sp] * sizeof(double), st));
+          if (ts[j].na[sp] > 0)
+            k_unsort_lp<<<nb(ts[j].na[sp]), kBlock, 0, st>>>(
+                ts[j].na[sp], perm[2 * (size_t)j + sp], lp[sp], lq[sp]);
+          std::swap(lp[sp], lq[sp]);
+        }
+      TRY(hipGetLastError() == hipSuccess ? 0 : set_error(ctx, PIC_E_HIP, "adjoint unsort"));
     }
   }
   // the particles alive at the first step's start (loaded and injected

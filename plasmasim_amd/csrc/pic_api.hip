@@ -992,7 +992,7 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
   } while (0)
   for (int s = 0; s < nsteps; ++s) {
     TRY(pic_renew_grid(ctx));
-    if (!ctx->adj_nosort && ctx->c_sort > c.sort_every - 1) {
+    if (ctx->c_sort > c.sort_every - 1) {
       ctx->c_sort = 0;
       TRY(pic_sort(ctx, PIC_ION));
       TRY(pic_sort(ctx, PIC_ELECTRON));

@@ -2205,7 +2205,7 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   // slots kept by the adjoint's forward (it needs the batch contiguous).
   // With one z tile layer both faces' slabs lie in the same tiles, and two
   // workgroups would fill the same dead slots: such tilings append.
-  const bool in_tile = add > 0 && !ctx->adj_nosort && ctx->sorted_end[sp] > 0 &&
+  const bool in_tile = add > 0 && !ctx->adj_mode && ctx->sorted_end[sp] > 0 &&
                        ctx->sorted_end[sp] <= ctx->n[sp] && tg.ntiles() > 1 &&
                        (nface == 1 || tg.NTZ >= 2) &&
                        tg.TZ >= 2 && g.NZ % tg.TZ == 0 && g.NX % tg.TX == 0 &&
@@ -2327,7 +2327,7 @@ int sort(pic_ctx *ctx, int sp) {
   }
   Particles32 src = p32(ctx, sp, ctx->cur[sp]);
   if (nt > 1 && tg.TX * tg.TY * tg.TZ <= kTileCells && ctx->sorted_end[sp] > 0 &&
-      ctx->sorted_end[sp] <= n && !ctx->adj_nosort) {
+      ctx->sorted_end[sp] <= n) {
     bool done = false;
     int64_t valid = 0;
     int rc = tile_sort(ctx, sp, tg, &done, &valid);

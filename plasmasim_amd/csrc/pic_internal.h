@@ -150,7 +150,9 @@ struct pic_ctx {
 
   // adjoint: checkpoints, tape and adjoint planes (pic_adj.hip)
   void *adj = nullptr;
-  int adj_nosort = 0;  // the adjoint's forward runs without the sort
+  // adjoint sweeps: inflows append to the store, so that each batch of
+  // injected particles stays contiguous for the tape
+  int adj_mode = 0;
 
   // 3D Poisson correction (poisson_every > 0): phi (a field plane of the
   // form's type, guard-inclusive), the transform buffer rho (nx ny nz) and

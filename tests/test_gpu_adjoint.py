@@ -4,7 +4,8 @@ oracle adjoint and central finite differences.
 Bar: with the plain FP64 forward, the loss matches the oracle's to 1e-10
 and the gradient (dipole moment vector, inflow speed) matches the oracle's
 FP64 gradient to 1e-6 and central differences (h = 1e-7 p) to 1e-5; the
-gradient does not depend on the checkpoint interval.  With the FP32
+gradient does not depend on the checkpoint interval, also when the forward
+sorts inside and at the start of checkpoint segments.  With the FP32
 forward the gradient is within 2e-2 of the FP64 one (printed)."""
 import numpy as np
 import pytest
@@ -67,6 +68,42 @@ def test_gradient_independent_of_checkpoint_interval():
     g3 = gpu_grad(c, 3)[1]
     g6 = gpu_grad(c, 6)[1]
     assert np.allclose(g1, g6, rtol=1e-9, atol=0) and np.allclose(g3, g6, rtol=1e-9, atol=0)
+
+
+def test_fp64_adjoint_across_sorts():
+    """The forward sorts at its cadence and the backward carries the particle
+    adjoints back across each sort.  With a sort every 2 steps (inside the
+    checkpoint segments and at their starts) the gradient matches the
+    oracle's (whose adjoint forward keeps slots, so the two differ only in
+    the FP64 deposit order) and central differences, for any checkpoint
+    interval."""
+    c = cfg()
+    c.sort_every = 2
+    s = ps.Simulation(c)
+    t0, r0 = s.sort_stats()
+    loss = s.adj_forward(params(c), NSTEPS, 2, R, NQ)
+    t1, r1 = s.sort_stats()
+    g = s.adj_backward(1.0)
+    s.close()
+    assert (t1 + r1) - (t0 + r0) >= 2 * (NSTEPS // 2 - 1), (t0, r0, t1, r1)
+    lo, go = Oracle(c, threads=4).adj_gradient(params(c), NSTEPS, R, NQ)
+    print("sorted GPU FP64 loss", loss, "grad", g, "| oracle", lo, go)
+    assert abs(loss - lo) <= 1e-10 * abs(lo)
+    assert np.all(np.abs(g - go) <= 1e-6 * np.abs(go)), (g, go)
+    p = params(c)
+    s = ps.Simulation(c)
+    for i in range(4):
+        h = 1e-7 * abs(p[i])
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (s.adj_forward(pp, NSTEPS, NSTEPS, R, NQ) -
+              s.adj_forward(pm, NSTEPS, NSTEPS, R, NQ)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * abs(fd), (i, fd, g[i])
+    s.close()
+    for k in (1, 3, 6):
+        gk = gpu_grad(c, k)[1]
+        assert np.allclose(gk, g, rtol=1e-9, atol=0), (k, gk, g)
 
 
 def test_fp32_forward_gradient_near_fp64():
