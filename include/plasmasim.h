@@ -292,11 +292,12 @@ int pic_timing_reset(pic_ctx *ctx);
  * the inflow drift speed along z.
  *
  * pic_adj_forward sets B0 from M and V, loads the particles and runs nsteps
- * steps of the context's form with the sort off (particles keep their
- * slots; the capacity must hold every injected particle), checkpointing
- * the state every ckpt_every steps; it returns L.  pic_adj_backward
- * recomputes each segment from its checkpoint and reverses it in FP64
- * (transposed FDTD, guard copies, fold, gather, Boris step and deposit),
+ * steps of the context's form (the cell sort at its cadence; each inflow
+ * appended to the store, whose capacity must hold it), checkpointing the
+ * state every ckpt_every steps; it returns L.  pic_adj_backward recomputes
+ * each segment from its checkpoint, recording each sort's permutation, and
+ * reverses it in FP64 (transposed FDTD, guard copies, fold, gather, Boris
+ * step and deposit, the particle adjoints carried back across each sort),
  * returning dloss * dL/dp.  The context is left at the end of the last
  * recomputed segment.
  */
