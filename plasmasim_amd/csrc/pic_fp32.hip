@@ -1232,8 +1232,9 @@ __device__ inline void gather4(T *__restrict__ d, const T *__restrict__ s,
 // as a radix sort of (key, slot) pairs:
 //   1. each source block (a tile range of the last sort, or a tail chunk)
 //      counts the particles still in their tile and writes the others, in
-//      slot order, as one block per destination tile into that tile's
-//      mover list, recording (block's first slot, offset, count);
+//      slot order, as one block per destination tile into a mover pool
+//      shared by all tiles (the block reserved with one atomic), recording
+//      for the destination (block's first slot, pool offset, count);
 //   2. a scan of (stayers + movers) per tile gives the new tile starts;
 //   3. one workgroup per tile orders its mover blocks by first slot, so the
 //      sequence (blocks from lower slots, its own stayers, blocks from
@@ -1241,11 +1242,11 @@ __device__ inline void gather4(T *__restrict__ d, const T *__restrict__ s,
 //      counting sort over the tile's cells.
 // Waves take contiguous quarters of a sequence: each counts, one scan sets
 // every (wave, value) start, and each wave then places its quarter in
-// order, ranking the lanes that share a value by ballots.  A block with
-// more than kDestMax destination tiles, a tile receiving more than
-// kMoverCap movers or kDescCap blocks, or a tile of more than kTileCells
-// cells takes the radix sort instead.
-constexpr int kMoverCap = 4096;
+// order, ranking the lanes that share a value by ballots.  The pool holds
+// as many movers as the store holds particles, so it cannot overflow; a
+// block with more than kDestMax destination tiles, a tile receiving mover
+// blocks from more than kDescCap source blocks, or a tile of more than
+// kTileCells cells takes the radix sort instead.
 constexpr int kDescCap = 64;
 constexpr int kDestMax = 64;
 constexpr int kTileCells = 1024;
@@ -1308,7 +1309,8 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
                                                      const int32_t *__restrict__ cell,
                                                      uint32_t *stay, uint32_t *mcount,
                                                      uint32_t *ndesc, uint4 *desc,
-                                                     uint2 *movers, uint32_t *flag) {
+                                                     uint2 *movers, uint32_t pool_cap,
+                                                     uint32_t *flag) {
   __shared__ uint32_t tkey[kDestMax];
   __shared__ uint32_t wc[kBlock / 64][kDestMax];
   __shared__ unsigned int kept_w[kBlock / 64];
@@ -1382,11 +1384,12 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
     const uint32_t t = tkey[tid];
     uint32_t tot = 0;
     for (int w = 0; w < kBlock / 64; ++w) tot += wc[w][tid];
-    const uint32_t off = atomicAdd(&mcount[t], tot);
+    (void)atomicAdd(&mcount[t], tot);
+    const uint32_t off = atomicAdd(flag + 1, tot);  // the pool's fill
     const uint32_t k = atomicAdd(&ndesc[t], 1u);
     if (k < (uint32_t)kDescCap)
       desc[(int64_t)t * kDescCap + k] = make_uint4(s, off, tot, 0u);
-    if (k >= (uint32_t)kDescCap || off + tot > (uint32_t)kMoverCap) atomicOr(flag, 1u);
+    if (k >= (uint32_t)kDescCap || off + tot > pool_cap) atomicOr(flag, 1u);
     uint32_t run = off;
     for (int w = 0; w < kBlock / 64; ++w) {
       const uint32_t c = wc[w][tid];
@@ -1413,8 +1416,7 @@ __global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRu
     const int r = wave_rank(mv, (uint32_t)(mv ? h : 0), 6, cnt);
     if (mv) {
       const uint32_t pos = wc[wv][h] + (uint32_t)r;
-      if (pos < (uint32_t)kMoverCap)
-        movers[(int64_t)t * kMoverCap + pos] = make_uint2(i0 + (uint32_t)lane, cin);
+      if (pos < pool_cap) movers[pos] = make_uint2(i0 + (uint32_t)lane, cin);
     }
     // the lanes' reads above precede this update in the wave's LDS order
     if (mv && r == 0) wc[wv][h] += (uint32_t)cnt;
@@ -1430,7 +1432,7 @@ __global__ __launch_bounds__(1024) void k_tsort_scan(const uint32_t *stay,
   const int per = (nt + 1023) / 1024;
   const int a = threadIdx.x * per, b = min(a + per, nt);
   uint32_t sum = 0;
-  for (int i = a; i < b; ++i) sum += stay[i] + min(mcount[i], (uint32_t)kMoverCap);
+  for (int i = a; i < b; ++i) sum += stay[i] + mcount[i];
   part[threadIdx.x] = sum;
   __syncthreads();
   for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
@@ -1442,7 +1444,7 @@ __global__ __launch_bounds__(1024) void k_tsort_scan(const uint32_t *stay,
   uint32_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
   for (int i = a; i < b; ++i) {
     start[i] = run;
-    run += stay[i] + min(mcount[i], (uint32_t)kMoverCap);
+    run += stay[i] + mcount[i];
   }
   if (threadIdx.x == 1023) start[nt] = part[1023];
 }
@@ -1456,6 +1458,7 @@ __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
                                                      const uint32_t *__restrict__ ndesc,
                                                      const uint4 *__restrict__ desc,
                                                      const uint2 *__restrict__ movers,
+                                                     uint32_t pool_cap,
                                                      const uint32_t *__restrict__ start,
                                                      uint32_t *idx) {
   // the sequence as segments: first sequence index and source offset; the
@@ -1488,8 +1491,8 @@ __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
     // higher slots.  A block past the list's end overflowed: the result is
     // then discarded, but its clamped length keeps every read in the list.
     const uint4 y = lane < nd ? d[lane] : make_uint4(kEmpty, 0u, 0u, 0u);
-    const uint32_t off = min(y.y, (uint32_t)kMoverCap);
-    const uint32_t ln = lane < nd ? min(y.z, (uint32_t)kMoverCap - off) : 0u;
+    const uint32_t off = min(y.y, pool_cap);
+    const uint32_t ln = lane < nd ? min(y.z, pool_cap - off) : 0u;
     const int own = __builtin_popcountll(__builtin_amdgcn_ballot_w64(lane < nd && y.x < s0));
     uint32_t inc = ln;  // inclusive scan over the wave
     for (int o = 1; o < 64; o <<= 1) {
@@ -1533,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
     while (sg + 1 < ns && seg_q[sg + 1] <= q) ++sg;
     const uint32_t r = q - seg_q[sg];
     if (sg != own) {
-      const uint2 m = movers[(int64_t)t * kMoverCap + seg_src[sg] + r];
+      const uint2 m = movers[seg_src[sg] + r];
       slot = m.x;
       c = m.y;
       return true;
@@ -1792,8 +1795,9 @@ struct State32 {
   float *bt = nullptr;               // total B = B0 + B1, 3 planes (+ 3 kept)
   bool fx_dirty = false;
   double fx_scale = 0;
-  // tile sort: per tile stayers, mover and block counts, block descriptors,
-  // mover lists and new starts; overflow flag
+  // tile sort: per tile stayers, mover and block counts, block descriptors
+  // and new starts; the mover pool (capacity entries); the overflow flag
+  // and the pool's fill
   uint32_t *ts_stay = nullptr, *ts_mcount = nullptr, *ts_ndesc = nullptr;
   uint4 *ts_desc = nullptr;
   uint2 *ts_movers = nullptr;
@@ -1844,6 +1848,29 @@ int refresh_total_b(pic_ctx *ctx) {
   return 0;
 }
 
+// the tile sort's lists for nt tiles (at set-up, so that no step allocates)
+static int tsort_buffers(pic_ctx *ctx, int nt) {
+  State32 *S = st(ctx);
+  if (S->ts_ntiles >= nt) return 0;
+  for (void **p : {(void **)&S->ts_stay, (void **)&S->ts_mcount, (void **)&S->ts_desc,
+                   (void **)&S->ts_movers, (void **)&S->ts_start, (void **)&S->ts_flag}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  S->ts_ndesc = nullptr;
+  S->ts_ntiles = 0;
+  PIC_HIP(ctx, hipMalloc(&S->ts_stay, (size_t)nt * sizeof(uint32_t)));
+  PIC_HIP(ctx, hipMalloc(&S->ts_mcount, (size_t)2 * nt * sizeof(uint32_t)));
+  S->ts_ndesc = S->ts_mcount + nt;
+  PIC_HIP(ctx, hipMalloc(&S->ts_desc, (size_t)nt * kDescCap * sizeof(uint4)));
+  PIC_HIP(ctx, hipMalloc(&S->ts_movers, (size_t)ctx->cap * sizeof(uint2)));
+  PIC_HIP(ctx, hipMalloc(&S->ts_start, (size_t)(nt + 1) * sizeof(uint32_t)));
+  // the overflow flag and the mover pool's fill
+  PIC_HIP(ctx, hipMalloc(&S->ts_flag, 2 * sizeof(uint32_t)));
+  S->ts_ntiles = nt;
+  return 0;
+}
+
 int init_state(pic_ctx *ctx) {
   State32 *S = new (std::nothrow) State32();
   if (!S) return set_error(ctx, PIC_E_NOMEM, "fp32 state");
@@ -1888,6 +1915,13 @@ int init_state(pic_ctx *ctx) {
   if (ctx->cfg.deposit == PIC_DEPOSIT_DETERMINISTIC) {
     PIC_HIP(ctx, hipMalloc(&S->fx, (size_t)8 * ctx->geom.plane * sizeof(unsigned long long)));
     PIC_HIP(ctx, hipMemset(S->fx, 0, (size_t)8 * ctx->geom.plane * sizeof(unsigned long long)));
+  }
+  {
+    const TileGeom tg = tile_geom(ctx);
+    if (tg.ntiles() > 1 && tg.TX * tg.TY * tg.TZ <= kTileCells) {
+      const int rc = tsort_buffers(ctx, tg.ntiles());
+      if (rc) return rc;
+    }
   }
   return 0;
 }
@@ -2252,7 +2286,8 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
 }
 
 // The tile sort (passes above) when the tiles of the last sort are current;
-// *done is false when a tile overflowed its mover list (the caller then
+// *done is false when a source block met more than kDestMax destination
+// tiles or a tile more than kDescCap source blocks (the caller then
 // takes the radix sort).  Leaves the permutation in ctx->idx, the new tile
 // starts in ctx->tile_start[sp] and the kept count in *valid.
 static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
@@ -2262,22 +2297,9 @@ static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
   const int nt = tg.ntiles();
   const int64_t n = ctx->n[sp];
   *done = false;
-  if (S->ts_ntiles < nt) {
-    for (void **p : {(void **)&S->ts_stay, (void **)&S->ts_mcount, (void **)&S->ts_ndesc,
-                     (void **)&S->ts_desc, (void **)&S->ts_movers, (void **)&S->ts_start,
-                     (void **)&S->ts_flag}) {
-      if (*p) (void)hipFree(*p);
-      *p = nullptr;
-    }
-    S->ts_ntiles = 0;
-    PIC_HIP(ctx, hipMalloc(&S->ts_stay, (size_t)nt * sizeof(uint32_t)));
-    PIC_HIP(ctx, hipMalloc(&S->ts_mcount, (size_t)2 * nt * sizeof(uint32_t)));
-    S->ts_ndesc = S->ts_mcount + nt;
-    PIC_HIP(ctx, hipMalloc(&S->ts_desc, (size_t)nt * kDescCap * sizeof(uint4)));
-    PIC_HIP(ctx, hipMalloc(&S->ts_movers, (size_t)nt * kMoverCap * sizeof(uint2)));
-    PIC_HIP(ctx, hipMalloc(&S->ts_start, (size_t)(nt + 1) * sizeof(uint32_t)));
-    PIC_HIP(ctx, hipMalloc(&S->ts_flag, sizeof(uint32_t)));
-    S->ts_ntiles = nt;
+  {
+    const int rc = tsort_buffers(ctx, nt);
+    if (rc) return rc;
   }
   TileRun tr;
   tr.tile_start = ctx->tile_start[sp];
@@ -2289,15 +2311,15 @@ static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
   // mover counts and descriptor counts (adjacent), and the flag
   k_tsort_clear<<<(unsigned)div_up(2 * nt, kBlock), kBlock, 0, ctx->stream>>>(S->ts_mcount,
                                                                          2 * nt);
-  k_tsort_clear<<<1, 64, 0, ctx->stream>>>(S->ts_flag, 1);
+  k_tsort_clear<<<1, 64, 0, ctx->stream>>>(S->ts_flag, 2);
   k_tsort_split<<<(unsigned)(nt + tr.ntail), kBlock, 0, ctx->stream>>>(
       g, tg, tr, src.cell, S->ts_stay, S->ts_mcount, S->ts_ndesc, S->ts_desc,
-      S->ts_movers, S->ts_flag);
+      S->ts_movers, (uint32_t)ctx->cap, S->ts_flag);
   k_tsort_scan<<<1, 1024, 0, ctx->stream>>>(S->ts_stay, S->ts_mcount, nt, S->ts_start);
   const size_t place_lds = (size_t)5 * tg.TX * tg.TY * tg.TZ * sizeof(uint32_t);
   k_tsort_place<<<(unsigned)nt, kBlock, place_lds, ctx->stream>>>(
       g, tg, ctx->tile_start[sp], src.cell, S->ts_ndesc, S->ts_desc, S->ts_movers,
-      S->ts_start, ctx->idx);
+      (uint32_t)ctx->cap, S->ts_start, ctx->idx);
   PIC_LAUNCH_CHECK(ctx);
   uint32_t *hc = (uint32_t *)ctx->host_counter;
   PIC_HIP(ctx, hipMemcpyAsync(hc, S->ts_start + nt, sizeof(uint32_t),
@@ -2305,7 +2327,7 @@ static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
   PIC_HIP(ctx, hipMemcpyAsync(hc + 1, S->ts_flag, sizeof(uint32_t),
                               hipMemcpyDeviceToHost, ctx->stream));
   PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (hc[1] != 0u) return 0;  // a list or a table overflowed: radix sort
+  if (hc[1] != 0u) return 0;  // a table overflowed: radix sort
   PIC_HIP(ctx, hipMemcpyAsync(ctx->tile_start[sp], S->ts_start,
                               (size_t)(nt + 1) * sizeof(uint32_t),
                               hipMemcpyDeviceToDevice, ctx->stream));

@@ -98,33 +98,60 @@ def test_tile_sort_after_steps_matches_oracle(tile):
     g.close()
 
 
-def test_tile_sort_falls_back_when_movers_overflow():
-    """20000 electrons in one 2 x 2 x 2 tile of the ballistic zone (drift
-    only) move at 0.9 c along z for three steps after a sort: more than a
-    tile's mover list holds enter the next tile, so the sort falls back to
-    the radix sort, and the result is still the oracle's stable sort."""
+def cluster(seed, vel):
+    """20000 electrons in one 2 x 2 x 2 tile of the ballistic layer (drift
+    only), velocities from vel(rng, n)."""
+    rng = np.random.default_rng(seed)
+    n = 20000
+    p = np.column_stack([rng.uniform(4, 6, n), rng.uniform(4, 6, n), rng.uniform(4, 6, n),
+                         vel(rng, n)])
+    cell = np.floor(p[:, :3])
+    p[:, :3] = cell + np.minimum(np.float32(p[:, :3] - cell).astype(np.float64), 0.99999994)
+    return p
+
+
+def sort_after(p, steps):
+    """(tile sorts, radix sorts) of the sort after `steps` steps from the
+    cluster's radix sort, and whether its result is the oracle's stable sort
+    of the same particles."""
     c = cfg(1, tile=(2, 2, 2))
     c.sort_every, c.inflow_every = 1000, 1000
     g = ps.Simulation(c)
-    rng = np.random.default_rng(9)
-    n = 20000
-    p = np.column_stack([rng.uniform(4, 6, n), rng.uniform(4, 6, n), rng.uniform(4, 6, n),
-                         np.zeros(n), np.zeros(n), np.full(n, 0.9)])
-    cell = np.floor(p[:, :3])
-    p[:, :3] = cell + np.minimum(np.float32(p[:, :3] - cell).astype(np.float64), 0.99999994)
     g.upload_particles(1, p)
     g.sort(1)  # uploaded, unsorted: radix sort; the tiles are current after it
-    g.step(3)
+    g.step(steps)
     pre = g.download_particles(1)
     t0, r0 = g.sort_stats()
     g.sort(1)
     t1, r1 = g.sort_stats()
-    assert (t1 - t0, r1 - r0) == (0, 1)
     o = Oracle(cfg(1, precision=ps.PIC_FP64, tile=(2, 2, 2)))
     o.upload_particles(1, pre)
     o.sort(1)
-    assert np.array_equal(g.download_particles(1), o.download_particles(1))
+    same = np.array_equal(g.download_particles(1), o.download_particles(1))
     g.close()
+    return (t1 - t0, r1 - r0), same
+
+
+def test_tile_sort_takes_a_crowd_of_movers():
+    """The cluster moves at 0.9 c along z for three steps: all 20000 enter
+    the next tile.  The movers share one pool sized by the capacity, so the
+    tile sort takes them, and the result is the oracle's stable sort."""
+    p = cluster(9, lambda rng, n: np.column_stack([np.zeros(n), np.zeros(n),
+                                                   np.full(n, 0.9)]))
+    counts, same = sort_after(p, 3)
+    assert counts == (1, 0) and same
+
+
+def test_tile_sort_falls_back_when_a_block_scatters():
+    """The cluster moves in random directions at up to 0.9 c for 16 steps:
+    its tile's particles reach more destination tiles than a source block's
+    table holds (64), so the sort falls back to the radix sort, and the
+    result is still the oracle's stable sort."""
+    def vel(rng, n):
+        d = rng.normal(size=(n, 3))
+        return 0.9 * d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0.5, 1, (n, 1))
+    counts, same = sort_after(cluster(11, vel), 16)
+    assert counts == (0, 1) and same
 
 
 def sims_with_state(f, p=None):
