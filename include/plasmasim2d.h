@@ -142,6 +142,9 @@ int pic2d_poisson(pic2d_ctx *ctx);                     /* renew_phi :2589 */
 int pic2d_particle_boundary(pic2d_ctx *ctx);           /* :812-901 */
 /* per-tile stable sort (no reference counterpart in the 2D loop) */
 int pic2d_sort(pic2d_ctx *ctx, int species);
+/* how many sorts so far took the tile sort (FP32 form, tiles current since
+ * the last sort) and the radix sort (no counterpart in the reference) */
+int pic2d_sort_stats(const pic2d_ctx *ctx, int64_t *tile, int64_t *radix);
 
 /* n whole steps of fdtd's loop with the particle loops and the particle
  * boundary fused into one pass per species */

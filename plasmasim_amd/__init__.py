@@ -363,6 +363,12 @@ class Simulation2D:
     def sort(self, sp):
         self._check(self._L.pic2d_sort(self._ctx, sp))
 
+    def sort_stats(self):
+        """(sorts by the tile sort, sorts by the radix sort) so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.pic2d_sort_stats(self._ctx, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def step(self, n=1):
         self._check(self._L.pic2d_step(self._ctx, int(n)))
 

@@ -212,6 +212,7 @@ SIGNATURES = [
     ("pic2d_poisson", ctypes.c_int, [_P]),
     ("pic2d_particle_boundary", ctypes.c_int, [_P]),
     ("pic2d_sort", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pic2d_sort_stats", ctypes.c_int, [_P, _I64P, _I64P]),
     ("pic2d_step", ctypes.c_int, [_P, ctypes.c_int]),
     ("pic2d_step_index", ctypes.c_int, [_P, _I64P]),
     ("pic2d_diagnose", ctypes.c_int, [_P, ctypes.POINTER(Pic2dDiag)]),

@@ -79,6 +79,7 @@ double depth2(int i2, int N, int A) {
 
 void free_all(pic2d_ctx *ctx) {
   pic_ctx &rt = ctx->rt;
+  tsort_free(ctx);
   if (ctx->field_mem) (void)hipFree(ctx->field_mem);
   if (ctx->sig64) (void)hipFree(ctx->sig64);
   if (ctx->sig32) (void)hipFree(ctx->sig32);
@@ -355,6 +356,7 @@ extern "C" int pic2d_create(const pic2d_config *cfg, pic2d_ctx **out) {
   CREATE_HIP(hipStreamSynchronize(rt.stream));
 #undef CREATE_HIP
   rc = init_poisson(ctx);
+  if (!rc) rc = tsort_setup(ctx);
   if (rc) {
     fprintf(stderr, "pic2d_create: %s\n", rt.err);
     free_all(ctx);
@@ -554,6 +556,13 @@ extern "C" int pic2d_sort(pic2d_ctx *ctx, int species) {
   int rc = sort(ctx, species);
   pic::timing_end(&ctx->rt, PIC_T_SORT);
   return rc;
+}
+
+extern "C" int pic2d_sort_stats(const pic2d_ctx *ctx, int64_t *tile, int64_t *radix) {
+  if (!ctx) return PIC_E_ARG;
+  if (tile) *tile = ctx->sort_tile_count;
+  if (radix) *radix = ctx->sort_radix_count;
+  return 0;
 }
 
 // pre-loop sequence (:584-626)

@@ -57,6 +57,12 @@ struct Push2 {
 
 }  // namespace pic2d
 
+namespace pic {
+namespace tsort {
+struct Bufs;
+}
+}  // namespace pic
+
 struct pic2d_ctx {
   pic_ctx rt;  // stream, error text, scratch, timing (its cfg is unused)
   pic2d_config cfg;
@@ -86,6 +92,11 @@ struct pic2d_ctx {
   int64_t step = 0;
   int32_t c_add = 0, c_sort = 0;
   uint32_t inflow_event = 0;
+
+  // FP32 form: the tile sort's device lists (pic_tsort.h) and how many
+  // sorts took the tile sort and the radix sort
+  pic::tsort::Bufs *ts = nullptr;
+  int64_t sort_tile_count = 0, sort_radix_count = 0;
 };
 
 namespace pic2d {
@@ -132,6 +143,8 @@ int push(pic2d_ctx *, int sp, int push_j, int drift_n, int boundary,
 int particle_boundary(pic2d_ctx *, int sp);
 int inflow(pic2d_ctx *, int sp, uint32_t event);
 int sort(pic2d_ctx *, int sp);
+int tsort_setup(pic2d_ctx *);  // the tile sort's lists, at create
+void tsort_free(pic2d_ctx *);
 int count_live(pic2d_ctx *, int sp, int64_t *n);
 int upload_particles(pic2d_ctx *, int sp, int64_t n, const double *const *h);
 int download_particles(pic2d_ctx *, int sp, int64_t cap, int64_t *n,

@@ -11,6 +11,7 @@
 // 64-bit fixed-point integers (on gfx950 integer LDS atomics are an order
 // of magnitude faster than fp32 ones) and flushes fp32 values.
 #include <hip/hip_runtime.h>
+#include <new>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -19,6 +20,7 @@
 
 #include "philox.h"
 #include "pic2d_internal.h"
+#include "pic_tsort.h"
 #include "pic_tiles.h"
 
 namespace pic2d {
@@ -1143,6 +1145,44 @@ __global__ void k_keys2(Geom2 g, Tile2 tg, int f64, P64 P, P32 Q, int64_t n,
   keys[i] = key;
 }
 
+// The tile sort's destination map of the 2D form (pic_tsort.h): cell index
+// k * SZ + m, key = tile * TX * TZ + (k % TX) * TZ + m % TZ over interior
+// cells, home slot from the tile coordinates modulo 8 (the 9 tiles around
+// any tile take distinct slots).
+struct KeyMap2 {
+  Geom2 g;
+  Tile2 tg;
+  __host__ __device__ uint32_t cells() const { return (uint32_t)(tg.TX * tg.TZ); }
+  __device__ bool dest(int32_t c, uint32_t &t, uint32_t &cin, int &h) const {
+    if (c < 0) return false;
+    const int k = c / g.SZ - 2, m = c % g.SZ - 2;
+    if (k < 0 || k >= g.NX || m < 0 || m >= g.NZ) return false;
+    const int tx = k / tg.TX, tz = m / tg.TZ;
+    t = (uint32_t)tx * tg.NTZ + (uint32_t)tz;
+    cin = (uint32_t)(k - tx * tg.TX) * tg.TZ + (uint32_t)(m - tz * tg.TZ);
+    h = ((tx & 7) << 3) | (tz & 7);
+    return true;
+  }
+};
+
+int tsort_setup(pic2d_ctx *ctx) {
+  const int nt = ctx->tg.ntiles();
+  // (slots are 32-bit in the tile sort: larger stores take the radix sort)
+  if (ctx->f64 || nt <= 1 || ctx->tg.TX * ctx->tg.TZ > pic::tsort::kTileCells ||
+      ctx->cap > ((int64_t)1 << 30))
+    return 0;
+  ctx->ts = new (std::nothrow) pic::tsort::Bufs();
+  if (!ctx->ts) return err(ctx, PIC_E_NOMEM, "tile sort lists");
+  return pic::tsort::reserve(&ctx->rt, *ctx->ts, nt, ctx->cap);
+}
+
+void tsort_free(pic2d_ctx *ctx) {
+  if (!ctx->ts) return;
+  pic::tsort::release(*ctx->ts);
+  delete ctx->ts;
+  ctx->ts = nullptr;
+}
+
 int sort(pic2d_ctx *ctx, int sp) {
   const int64_t n = ctx->n[sp];
   const int nt = ctx->tg.ntiles();
@@ -1150,6 +1190,29 @@ int sort(pic2d_ctx *ctx, int sp) {
   if (n == 0) return clear_tiles(ctx, sp);
   const int src = ctx->cur[sp], dst = 1 - src;
   hipStream_t st = ctx->rt.stream;
+  if (ctx->ts && ctx->sorted_end[sp] > 0 && ctx->sorted_end[sp] <= n) {
+    // the tiles of the last sort are current: the tile sort
+    const KeyMap2 km{ctx->g, ctx->tg};
+    bool done = false;
+    int64_t valid = 0;
+    int rc = pic::tsort::run(&ctx->rt, km, nt, ctx->tile_start[sp], ctx->sorted_end[sp], n,
+                             p32(ctx, sp, src).cell, *ctx->ts, ctx->rt.idx, &done, &valid);
+    if (rc) return rc;
+    if (done) {
+      if (valid > 0) {
+        k_gather2<<<(unsigned)div_up(valid, kBlock), kBlock, 0, st>>>(
+            f64, p64(ctx, sp, dst), p32(ctx, sp, dst), p64(ctx, sp, src),
+            p32(ctx, sp, src), ctx->rt.idx, valid);
+        P2_LAUNCH_CHECK(ctx);
+      }
+      ctx->cur[sp] = dst;
+      ctx->n[sp] = valid;
+      ctx->sorted_end[sp] = valid;
+      ctx->sort_tile_count++;
+      return 0;
+    }
+  }
+  ctx->sort_radix_count++;
   k_keys2<<<(unsigned)div_up(n, kBlock), kBlock, 0, st>>>(
       ctx->g, ctx->tg, f64, p64(ctx, sp, src), p32(ctx, sp, src), n,
       ctx->cfg.dx, ctx->rt.keys);

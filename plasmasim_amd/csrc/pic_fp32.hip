@@ -27,6 +27,7 @@
 #include "philox.h"
 #include "pic_internal.h"
 #include "pic_tiles.h"
+#include "pic_tsort.h"
 
 namespace pic {
 namespace f32 {
@@ -1225,385 +1226,40 @@ __device__ inline void gather4(T *__restrict__ d, const T *__restrict__ s,
 
 
 // ---------------------------------------------------------------------------
-// tile sort (sorting :7475-7590 with the tile-major key), written for the
-// state a periodic sort meets: the particles are still tile-sorted from the
-// last sort except for a small appended tail, and most of them are still in
-// their tile.  The result is the stable sort by key, the same permutation
-// as a radix sort of (key, slot) pairs:
-//   1. each source block (a tile range of the last sort, or a tail chunk)
-//      counts the particles still in their tile and writes the others, in
-//      slot order, as one block per destination tile into a mover pool
-//      shared by all tiles (the block reserved with one atomic), recording
-//      for the destination (block's first slot, pool offset, count);
-//   2. a scan of (stayers + movers) per tile gives the new tile starts;
-//   3. one workgroup per tile orders its mover blocks by first slot, so the
-//      sequence (blocks from lower slots, its own stayers, blocks from
-//      higher slots) is in slot order, and places that sequence by a stable
-//      counting sort over the tile's cells.
-// Waves take contiguous quarters of a sequence: each counts, one scan sets
-// every (wave, value) start, and each wave then places its quarter in
-// order, ranking the lanes that share a value by ballots.  The pool holds
-// as many movers as the store holds particles, so it cannot overflow; a
-// block with more than kDestMax destination tiles, a tile receiving mover
-// blocks from more than kDescCap source blocks, or a tile of more than
-// kTileCells cells takes the radix sort instead.
-constexpr int kDescCap = 64;
-constexpr int kDestMax = 64;
-constexpr int kTileCells = 1024;
-constexpr uint32_t kEmpty = 0xffffffffu;
-// tail particles per source block: the inflow overflow is appended tile by
-// tile, a few to some tens per slab tile, so a short block meets few tiles
-constexpr int kSortTail = 256;
-
-// The destination of a stored particle: false when it dropped out, else its
+// tile sort (the passes are in pic_tsort.h): the 3D destination map.  The
+// destination of a stored particle: false when it dropped out, else its
 // key's tile t and cell in the tile cin (key = t * cells per tile + cin) and
-// h, its tile's home slot in a 64-entry table: the tile coordinates modulo
-// 4, so that the 27 tiles around any tile take distinct slots.
-__device__ inline bool tsort_dest(const Geom &g, const TileGeom &tg, int32_t c, uint32_t &t,
-                                  uint32_t &cin, int &h) {
-  if (c < 0) return false;
-  int k, l, m;
-  unpack(c, k, l, m);
-  k -= 2;
-  l -= 2;
-  m -= 2;
-  if (k < 0 || k >= g.NX || l < 0 || l >= g.NY || m < 0 || m >= g.NZ) return false;
-  int tx, ty, tz;
-  if (tg.LX >= 0) {
-    tx = k >> tg.LX;
-    ty = l >> tg.LY;
-    tz = m >> tg.LZ;
-  } else {
-    tx = k / tg.TX;
-    ty = l / tg.TY;
-    tz = m / tg.TZ;
-  }
-  const int ix = k - tx * tg.TX, iy = l - ty * tg.TY, iz = m - tz * tg.TZ;
-  t = ((uint32_t)tx * tg.NTY + (uint32_t)ty) * tg.NTZ + (uint32_t)tz;
-  cin = ((uint32_t)ix * tg.TY + (uint32_t)iy) * tg.TZ + (uint32_t)iz;
-  h = ((tx & 3) << 4) | ((ty & 3) << 2) | (tz & 3);
-  return true;
-}
-
-// rank of this lane among the active lanes (pred) of its wave holding the
-// same value v (nbits wide), and the number of such lanes
-__device__ inline int wave_rank(bool pred, uint32_t v, int nbits, int &count) {
-  const int lane = threadIdx.x & 63;
-  u64 same = __builtin_amdgcn_ballot_w64(pred);
-  for (int b = 0; b < nbits; ++b) {
-    const u64 on = __builtin_amdgcn_ballot_w64(pred && ((v >> b) & 1u));
-    same &= ((v >> b) & 1u) ? on : ~on;
-  }
-  count = __builtin_popcountll(same);
-  const u64 lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  return __builtin_popcountll(same & lt);
-}
-
-__global__ void k_tsort_clear(uint32_t *a, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0u;
-}
-
-// pass 1: one workgroup per tile range of the last sort or tail chunk
-__global__ __launch_bounds__(256) void k_tsort_split(Geom g, TileGeom tg, TileRun tr,
-                                                     const int32_t *__restrict__ cell,
-                                                     uint32_t *stay, uint32_t *mcount,
-                                                     uint32_t *ndesc, uint4 *desc,
-                                                     uint2 *movers, uint32_t pool_cap,
-                                                     uint32_t *flag) {
-  __shared__ uint32_t tkey[kDestMax];
-  __shared__ uint32_t wc[kBlock / 64][kDestMax];
-  __shared__ unsigned int kept_w[kBlock / 64];
-  const int bid = blockIdx.x, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  const bool own_tile = bid < tr.ntiles;
-  // slots fit 32 bits: the library keeps at most 2^30 particles per species
-  uint32_t s, e;
-  if (own_tile) {
-    s = tr.tile_start[bid];
-    e = tr.tile_start[bid + 1];
-  } else {
-    s = (uint32_t)tr.sorted_end + (uint32_t)(bid - tr.ntiles) * kSortTail;
-    e = min(s + (uint32_t)kSortTail, (uint32_t)tr.n);
-  }
-  for (int h = tid; h < kDestMax; h += kBlock) {
-    tkey[h] = kEmpty;
-    for (int w = 0; w < kBlock / 64; ++w) wc[w][h] = 0u;
-  }
-  __syncthreads();
-  // this wave's contiguous quarter of the block
-  const uint32_t per = (e - s + 3) / 4;
-  const uint32_t q0 = min(s + wv * per, e), q1 = min(q0 + per, e);
-  // the destination's table slot by linear probing from its home slot h,
-  // -1 when the table is full (insert) or the tile is absent (find)
-  const auto slot_of = [&](uint32_t t, int h, bool insert) -> int {
-    for (int i = 0; i < kDestMax; ++i) {
-      const uint32_t cur = tkey[h];
-      if (cur == t) return h;
-      if (cur == kEmpty) {
-        if (!insert) return -1;
-        const uint32_t old = atomicCAS(&tkey[h], kEmpty, t);
-        if (old == kEmpty || old == t) return h;
-      }
-      h = (h + 1) & (kDestMax - 1);
+// h, its tile's home slot in the sort's 64-entry table: the tile coordinates
+// modulo 4, so that the 27 tiles around any tile take distinct slots.
+struct KeyMap3 {
+  Geom g;
+  TileGeom tg;
+  __host__ __device__ uint32_t cells() const { return (uint32_t)(tg.TX * tg.TY * tg.TZ); }
+  __device__ bool dest(int32_t c, uint32_t &t, uint32_t &cin, int &h) const {
+    if (c < 0) return false;
+    int k, l, m;
+    unpack(c, k, l, m);
+    k -= 2;
+    l -= 2;
+    m -= 2;
+    if (k < 0 || k >= g.NX || l < 0 || l >= g.NY || m < 0 || m >= g.NZ) return false;
+    int tx, ty, tz;
+    if (tg.LX >= 0) {
+      tx = k >> tg.LX;
+      ty = l >> tg.LY;
+      tz = m >> tg.LZ;
+    } else {
+      tx = k / tg.TX;
+      ty = l / tg.TY;
+      tz = m / tg.TZ;
     }
-    return -1;
-  };
-  // a particle's table slot when it moves to another tile, else -1
-  const auto classify = [&](int32_t c, bool insert, uint32_t &t, uint32_t &cin,
-                            bool &kept) -> int {
-    int h0;
-    kept = false;
-    if (!tsort_dest(g, tg, c, t, cin, h0)) return -1;
-    if (own_tile && t == (uint32_t)bid) {
-      kept = true;
-      return -1;
-    }
-    const int h = slot_of(t, h0, insert);
-    if (h < 0 && insert) atomicOr(flag, 1u);
-    return h;
-  };
-  unsigned int kept = 0;
-  int32_t cnext = q0 + lane < q1 ? cell[q0 + lane] : -1;
-  for (uint32_t i0 = q0; i0 < q1; i0 += 64) {
-    const int32_t c = cnext;
-    cnext = i0 + 64 + lane < q1 ? cell[i0 + 64 + lane] : -1;
-    uint32_t t = 0, cin = 0;
-    bool kp;
-    const int h = classify(c, true, t, cin, kp);
-    kept += kp;
-    // one LDS add per destination present in the wave
-    int cnt;
-    const int r = wave_rank(h >= 0, (uint32_t)(h >= 0 ? h : 0), 6, cnt);
-    if (h >= 0 && r == 0) atomicAdd(&wc[wv][h], (uint32_t)cnt);
+    const int ix = k - tx * tg.TX, iy = l - ty * tg.TY, iz = m - tz * tg.TZ;
+    t = ((uint32_t)tx * tg.NTY + (uint32_t)ty) * tg.NTZ + (uint32_t)tz;
+    cin = ((uint32_t)ix * tg.TY + (uint32_t)iy) * tg.TZ + (uint32_t)iz;
+    h = ((tx & 3) << 4) | ((ty & 3) << 2) | (tz & 3);
+    return true;
   }
-  for (int o = 32; o > 0; o >>= 1) kept += __shfl_xor(kept, o);
-  if (lane == 0) kept_w[wv] = kept;
-  __syncthreads();
-  // one reservation per destination: its total, its offset, one descriptor
-  if (tid < kDestMax && tkey[tid] != kEmpty) {
-    const uint32_t t = tkey[tid];
-    uint32_t tot = 0;
-    for (int w = 0; w < kBlock / 64; ++w) tot += wc[w][tid];
-    (void)atomicAdd(&mcount[t], tot);
-    const uint32_t off = atomicAdd(flag + 1, tot);  // the pool's fill
-    const uint32_t k = atomicAdd(&ndesc[t], 1u);
-    if (k < (uint32_t)kDescCap)
-      desc[(int64_t)t * kDescCap + k] = make_uint4(s, off, tot, 0u);
-    if (k >= (uint32_t)kDescCap || off + tot > pool_cap) atomicOr(flag, 1u);
-    uint32_t run = off;
-    for (int w = 0; w < kBlock / 64; ++w) {
-      const uint32_t c = wc[w][tid];
-      wc[w][tid] = run;
-      run += c;
-    }
-  }
-  if (tid == 0 && own_tile) {
-    unsigned int k = 0;
-    for (int w = 0; w < kBlock / 64; ++w) k += kept_w[w];
-    stay[bid] = k;
-  }
-  __syncthreads();
-  // each wave writes its movers in slot order
-  cnext = q0 + lane < q1 ? cell[q0 + lane] : -1;
-  for (uint32_t i0 = q0; i0 < q1; i0 += 64) {
-    const int32_t c = cnext;
-    cnext = i0 + 64 + lane < q1 ? cell[i0 + 64 + lane] : -1;
-    uint32_t t = 0, cin = 0;
-    bool kp;
-    const int h = classify(c, false, t, cin, kp);
-    const bool mv = h >= 0;
-    int cnt;
-    const int r = wave_rank(mv, (uint32_t)(mv ? h : 0), 6, cnt);
-    if (mv) {
-      const uint32_t pos = wc[wv][h] + (uint32_t)r;
-      if (pos < pool_cap) movers[pos] = make_uint2(i0 + (uint32_t)lane, cin);
-    }
-    // the lanes' reads above precede this update in the wave's LDS order
-    if (mv && r == 0) wc[wv][h] += (uint32_t)cnt;
-  }
-}
-
-// pass 2: start[t] = exclusive scan of stay[t] + mcount[t] over the tiles,
-// start[nt] = the kept total (one workgroup of 1024 threads)
-__global__ __launch_bounds__(1024) void k_tsort_scan(const uint32_t *stay,
-                                                     const uint32_t *mcount, int nt,
-                                                     uint32_t *start) {
-  __shared__ uint32_t part[1024];
-  const int per = (nt + 1023) / 1024;
-  const int a = threadIdx.x * per, b = min(a + per, nt);
-  uint32_t sum = 0;
-  for (int i = a; i < b; ++i) sum += stay[i] + mcount[i];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
-    const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0u;
-  for (int i = a; i < b; ++i) {
-    start[i] = run;
-    run += stay[i] + mcount[i];
-  }
-  if (threadIdx.x == 1023) start[nt] = part[1023];
-}
-
-// pass 3: one workgroup per tile t; writes idx[start[t] + position] = slot.
-// Dynamic LDS: per-wave cell counters (4 x cells per tile) and the cells'
-// totals (cells per tile), 20 bytes per cell.
-__global__ __launch_bounds__(256) void k_tsort_place(Geom g, TileGeom tg,
-                                                     const uint32_t *__restrict__ old_start,
-                                                     const int32_t *__restrict__ cell,
-                                                     const uint32_t *__restrict__ ndesc,
-                                                     const uint4 *__restrict__ desc,
-                                                     const uint2 *__restrict__ movers,
-                                                     uint32_t pool_cap,
-                                                     const uint32_t *__restrict__ start,
-                                                     uint32_t *idx) {
-  // the sequence as segments: first sequence index and source offset; the
-  // segment s_own is the tile's own range from slot s0, the others are
-  // mover blocks at that offset of the tile's list
-  __shared__ uint32_t seg_q[kDescCap + 2], seg_src[kDescCap + 2];
-  __shared__ int s_own;
-  __shared__ uint4 d[kDescCap];
-  __shared__ uint32_t part[kBlock];
-  extern __shared__ uint32_t place_lds[];
-  const int t = blockIdx.x, tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  const uint32_t cpt = (uint32_t)(tg.TX * tg.TY * tg.TZ);
-  uint32_t *cnt = place_lds + wv * cpt;  // this wave's counters
-  uint32_t *tot = place_lds + 4 * cpt;
-  const uint32_t s0 = old_start[t], len = old_start[t + 1] - s0;
-  const int nd = (int)min(ndesc[t], (uint32_t)kDescCap);
-  if (wv == 0) {
-    // the mover blocks in order of first slot: the first slots differ, so
-    // each block's rank is the number of blocks with a smaller one
-    uint4 x = make_uint4(kEmpty, 0u, 0u, 0u);
-    if (lane < nd) x = desc[(int64_t)t * kDescCap + lane];
-    int rank = 0;
-    for (int j = 0; j < nd; ++j) rank += (uint32_t)__builtin_amdgcn_readlane((int)x.x, j) < x.x;
-    if (lane < nd) d[rank] = x;
-  }
-  for (uint32_t j = tid; j < 4 * cpt; j += kBlock) place_lds[j] = 0u;
-  __syncthreads();
-  if (wv == 0) {
-    // the sequence: blocks from lower slots, the own range, blocks from
-    // higher slots.  A block past the list's end overflowed: the result is
-    // then discarded, but its clamped length keeps every read in the list.
-    const uint4 y = lane < nd ? d[lane] : make_uint4(kEmpty, 0u, 0u, 0u);
-    const uint32_t off = min(y.y, pool_cap);
-    const uint32_t ln = lane < nd ? min(y.z, pool_cap - off) : 0u;
-    const int own = __builtin_popcountll(__builtin_amdgcn_ballot_w64(lane < nd && y.x < s0));
-    uint32_t inc = ln;  // inclusive scan over the wave
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t v = __shfl_up(inc, o);
-      if (lane >= o) inc += v;
-    }
-    const uint32_t below = own > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)inc, own - 1) : 0u;
-    const uint32_t sum = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-    if (lane < nd) {
-      const int si = lane < own ? lane : lane + 1;
-      seg_q[si] = inc - ln + (lane < own ? 0u : len);
-      seg_src[si] = y.y;
-    }
-    if (lane == 0) {
-      seg_q[own] = below;
-      seg_src[own] = s0;
-      seg_q[nd + 1] = sum + len;  // end of the sequence
-      s_own = own;
-    }
-  }
-  __syncthreads();
-  const int ns = nd + 1, own = s_own;
-  const uint32_t total = seg_q[ns];
-  const uint32_t per = (total + 3) / 4;
-  const uint32_t q0 = min(wv * per, total), q1 = min(q0 + per, total);
-  // this lane's segment: the last one starting at or before q, found once
-  // by bisection, then followed as q grows
-  const auto seg_find = [&](uint32_t q) -> int {
-    int lo = 0, hi = ns - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (seg_q[mid] <= q) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-  };
-  const int sg0 = seg_find(q0 + lane);
-  int sg = sg0;
-  // element q of the sequence: (slot, cell in the tile), false for a range
-  // slot whose particle left the tile or dropped out
-  const auto elem = [&](uint32_t q, uint32_t &slot, uint32_t &c) -> bool {
-    while (sg + 1 < ns && seg_q[sg + 1] <= q) ++sg;
-    const uint32_t r = q - seg_q[sg];
-    if (sg != own) {
-      const uint2 m = movers[seg_src[sg] + r];
-      slot = m.x;
-      c = m.y;
-      return true;
-    }
-    slot = s0 + r;
-    uint32_t tt;
-    int h;
-    return tsort_dest(g, tg, cell[slot], tt, c, h) && tt == (uint32_t)t;
-  };
-  for (uint32_t qb = q0; qb < q1; qb += 64) {
-    const uint32_t q = qb + lane;
-    uint32_t slot, c = 0;
-    const bool ok = q < q1 && elem(q, slot, c);
-    // lanes in runs of one cell: the run's first lane adds the run's length
-    const uint32_t cv = ok ? c : kEmpty;
-    const uint32_t cp = __shfl_up(cv, 1);
-    const bool first = lane == 0 || cv != cp;
-    const u64 heads = __builtin_amdgcn_ballot_w64(first);
-    if (ok && first) {
-      const u64 above = (heads >> lane) >> 1;
-      const uint32_t run = above ? (uint32_t)__builtin_ctzll(above) + 1u : 64u - lane;
-      atomicAdd(&cnt[c], run);
-    }
-  }
-  __syncthreads();
-  // cell starts (exclusive scan over the cells' totals), then each wave's
-  // start per cell
-  for (uint32_t c = tid; c < cpt; c += kBlock) {
-    uint32_t v = 0;
-    for (int w = 0; w < kBlock / 64; ++w) v += place_lds[w * cpt + c];
-    tot[c] = v;
-  }
-  __syncthreads();
-  const uint32_t cper = (cpt + kBlock - 1) / kBlock;
-  const uint32_t ca = min(tid * cper, cpt), cb = min(ca + cper, cpt);
-  uint32_t sum = 0;
-  for (uint32_t c = ca; c < cb; ++c) sum += tot[c];
-  part[tid] = sum;
-  __syncthreads();
-  for (int o = 1; o < kBlock; o <<= 1) {  // inclusive Hillis-Steele scan
-    const uint32_t v = tid >= o ? part[tid - o] : 0u;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  uint32_t run = start[t] + (tid > 0 ? part[tid - 1] : 0u);
-  for (uint32_t c = ca; c < cb; ++c) {
-    for (int w = 0; w < kBlock / 64; ++w) {
-      const uint32_t v = place_lds[w * cpt + c];
-      place_lds[w * cpt + c] = run;
-      run += v;
-    }
-  }
-  __syncthreads();
-  int nbits = 1;
-  while ((1u << nbits) < cpt) ++nbits;
-  sg = sg0;
-  for (uint32_t qb = q0; qb < q1; qb += 64) {
-    const uint32_t q = qb + lane;
-    uint32_t slot = 0, c = 0;
-    const bool ok = q < q1 && elem(q, slot, c);
-    int n;
-    const int r = wave_rank(ok, c, nbits, n);
-    if (ok) idx[cnt[c] + (uint32_t)r] = slot;
-    if (ok && r == 0) cnt[c] += (uint32_t)n;
-  }
-}
+};
 
 __global__ void k_gather32(Particles32 dst, Particles32 src,
                            const uint32_t *__restrict__ idx, int64_t n) {
@@ -1795,14 +1451,7 @@ struct State32 {
   float *bt = nullptr;               // total B = B0 + B1, 3 planes (+ 3 kept)
   bool fx_dirty = false;
   double fx_scale = 0;
-  // tile sort: per tile stayers, mover and block counts, block descriptors
-  // and new starts; the mover pool (capacity entries); the overflow flag
-  // and the pool's fill
-  uint32_t *ts_stay = nullptr, *ts_mcount = nullptr, *ts_ndesc = nullptr;
-  uint4 *ts_desc = nullptr;
-  uint2 *ts_movers = nullptr;
-  uint32_t *ts_start = nullptr, *ts_flag = nullptr;
-  int ts_ntiles = 0;
+  tsort::Bufs ts;  // the tile sort's lists (mover pool of capacity entries)
 };
 
 static State32 *st(pic_ctx *ctx) { return (State32 *)ctx->form_state; }
@@ -1845,29 +1494,6 @@ int refresh_total_b(pic_ctx *ctx) {
                                                  st(ctx)->bt, g.plane, n);
   PIC_LAUNCH_CHECK(ctx);
   PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return 0;
-}
-
-// the tile sort's lists for nt tiles (at set-up, so that no step allocates)
-static int tsort_buffers(pic_ctx *ctx, int nt) {
-  State32 *S = st(ctx);
-  if (S->ts_ntiles >= nt) return 0;
-  for (void **p : {(void **)&S->ts_stay, (void **)&S->ts_mcount, (void **)&S->ts_desc,
-                   (void **)&S->ts_movers, (void **)&S->ts_start, (void **)&S->ts_flag}) {
-    if (*p) (void)hipFree(*p);
-    *p = nullptr;
-  }
-  S->ts_ndesc = nullptr;
-  S->ts_ntiles = 0;
-  PIC_HIP(ctx, hipMalloc(&S->ts_stay, (size_t)nt * sizeof(uint32_t)));
-  PIC_HIP(ctx, hipMalloc(&S->ts_mcount, (size_t)2 * nt * sizeof(uint32_t)));
-  S->ts_ndesc = S->ts_mcount + nt;
-  PIC_HIP(ctx, hipMalloc(&S->ts_desc, (size_t)nt * kDescCap * sizeof(uint4)));
-  PIC_HIP(ctx, hipMalloc(&S->ts_movers, (size_t)ctx->cap * sizeof(uint2)));
-  PIC_HIP(ctx, hipMalloc(&S->ts_start, (size_t)(nt + 1) * sizeof(uint32_t)));
-  // the overflow flag and the mover pool's fill
-  PIC_HIP(ctx, hipMalloc(&S->ts_flag, 2 * sizeof(uint32_t)));
-  S->ts_ntiles = nt;
   return 0;
 }
 
@@ -1918,8 +1544,9 @@ int init_state(pic_ctx *ctx) {
   }
   {
     const TileGeom tg = tile_geom(ctx);
-    if (tg.ntiles() > 1 && tg.TX * tg.TY * tg.TZ <= kTileCells) {
-      const int rc = tsort_buffers(ctx, tg.ntiles());
+    // the tile sort's lists, at set-up so that no step allocates
+    if (tg.ntiles() > 1 && tg.TX * tg.TY * tg.TZ <= tsort::kTileCells) {
+      const int rc = tsort::reserve(ctx, S->ts, tg.ntiles(), ctx->cap);
       if (rc) return rc;
     }
   }
@@ -1932,9 +1559,7 @@ void free_state(pic_ctx *ctx) {
   if (S->sig32) (void)hipFree(S->sig32);
   if (S->fx) (void)hipFree(S->fx);
   if (S->bt) (void)hipFree(S->bt);
-  for (void *p : {(void *)S->ts_stay, (void *)S->ts_mcount, (void *)S->ts_desc,
-                  (void *)S->ts_movers, (void *)S->ts_start, (void *)S->ts_flag})
-    if (p) (void)hipFree(p);
+  tsort::release(S->ts);
   delete S;
   ctx->form_state = nullptr;
 }
@@ -2285,57 +1910,6 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   return 0;
 }
 
-// The tile sort (passes above) when the tiles of the last sort are current;
-// *done is false when a source block met more than kDestMax destination
-// tiles or a tile more than kDescCap source blocks (the caller then
-// takes the radix sort).  Leaves the permutation in ctx->idx, the new tile
-// starts in ctx->tile_start[sp] and the kept count in *valid.
-static int tile_sort(pic_ctx *ctx, int sp, const TileGeom &tg, bool *done,
-                     int64_t *valid) {
-  State32 *S = st(ctx);
-  const Geom &g = ctx->geom;
-  const int nt = tg.ntiles();
-  const int64_t n = ctx->n[sp];
-  *done = false;
-  {
-    const int rc = tsort_buffers(ctx, nt);
-    if (rc) return rc;
-  }
-  TileRun tr;
-  tr.tile_start = ctx->tile_start[sp];
-  tr.sorted_end = ctx->sorted_end[sp];
-  tr.ntiles = nt;
-  tr.n = n;
-  tr.ntail = (int)div_up(n - tr.sorted_end, kSortTail);
-  const Particles32 src = p32(ctx, sp, ctx->cur[sp]);
-  // mover counts and descriptor counts (adjacent), and the flag
-  k_tsort_clear<<<(unsigned)div_up(2 * nt, kBlock), kBlock, 0, ctx->stream>>>(S->ts_mcount,
-                                                                         2 * nt);
-  k_tsort_clear<<<1, 64, 0, ctx->stream>>>(S->ts_flag, 2);
-  k_tsort_split<<<(unsigned)(nt + tr.ntail), kBlock, 0, ctx->stream>>>(
-      g, tg, tr, src.cell, S->ts_stay, S->ts_mcount, S->ts_ndesc, S->ts_desc,
-      S->ts_movers, (uint32_t)ctx->cap, S->ts_flag);
-  k_tsort_scan<<<1, 1024, 0, ctx->stream>>>(S->ts_stay, S->ts_mcount, nt, S->ts_start);
-  const size_t place_lds = (size_t)5 * tg.TX * tg.TY * tg.TZ * sizeof(uint32_t);
-  k_tsort_place<<<(unsigned)nt, kBlock, place_lds, ctx->stream>>>(
-      g, tg, ctx->tile_start[sp], src.cell, S->ts_ndesc, S->ts_desc, S->ts_movers,
-      (uint32_t)ctx->cap, S->ts_start, ctx->idx);
-  PIC_LAUNCH_CHECK(ctx);
-  uint32_t *hc = (uint32_t *)ctx->host_counter;
-  PIC_HIP(ctx, hipMemcpyAsync(hc, S->ts_start + nt, sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  PIC_HIP(ctx, hipMemcpyAsync(hc + 1, S->ts_flag, sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (hc[1] != 0u) return 0;  // a table overflowed: radix sort
-  PIC_HIP(ctx, hipMemcpyAsync(ctx->tile_start[sp], S->ts_start,
-                              (size_t)(nt + 1) * sizeof(uint32_t),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-  *valid = (int64_t)hc[0];
-  *done = true;
-  return 0;
-}
-
 int sort(pic_ctx *ctx, int sp) {
   const Geom &g = ctx->geom;
   State32 *S = st(ctx);
@@ -2348,11 +1922,15 @@ int sort(pic_ctx *ctx, int sp) {
     return 0;
   }
   Particles32 src = p32(ctx, sp, ctx->cur[sp]);
-  if (nt > 1 && tg.TX * tg.TY * tg.TZ <= kTileCells && ctx->sorted_end[sp] > 0 &&
+  if (nt > 1 && tg.TX * tg.TY * tg.TZ <= tsort::kTileCells && ctx->sorted_end[sp] > 0 &&
       ctx->sorted_end[sp] <= n) {
     bool done = false;
     int64_t valid = 0;
-    int rc = tile_sort(ctx, sp, tg, &done, &valid);
+    const KeyMap3 km{g, tg};
+    int rc = tsort::reserve(ctx, S->ts, nt, ctx->cap);
+    if (!rc)
+      rc = tsort::run(ctx, km, nt, ctx->tile_start[sp], ctx->sorted_end[sp], n, src.cell,
+                      S->ts, ctx->idx, &done, &valid);
     if (rc) return rc;
     if (done) {
       Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);

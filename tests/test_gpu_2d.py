@@ -146,6 +146,32 @@ def test_inflow_and_sort_bit_exact():
         assert rel_l2(a[:, 2:], b[:, 2:]) < 1e-14
 
 
+def test_fp32_tile_sort_matches_oracle():
+    """FP32 form: the sort after 6 steps (the tiles of the load's sort still
+    current, the inflows of begin and of steps 3 and 6 appended as a tail)
+    takes the tile sort, and its result equals the oracle's stable sort of
+    the same particles (cell-major within 16 x 16 tiles, slot order within a
+    cell; the dropped ones removed)."""
+    c = cfg2(1, ps.PIC_FP32, sort_every=1000, inflow_every=3)
+    g = ps.Simulation2D(c)
+    g.load_particles()
+    g.begin()
+    g.step(6)
+    pre = [g.download_particles(sp) for sp in (0, 1)]
+    t0, r0 = g.sort_stats()
+    g.sort(0)
+    g.sort(1)
+    t1, r1 = g.sort_stats()
+    assert (t1 - t0, r1 - r0) == (2, 0)
+    o = Oracle2D(cfg2(1, ps.PIC_FP64, sort_every=1000, inflow_every=3))
+    for sp in (0, 1):
+        o.upload_particles(sp, pre[sp])
+        o.sort(sp)
+        a, b = g.download_particles(sp), o.download_particles(sp)
+        assert a.shape == b.shape and np.array_equal(a, b), sp
+    g.close()
+
+
 @pytest.mark.parametrize("sort_every", [0, 3])
 def test_whole_steps_match_oracle(sort_every):
     g, o, c = pair(sort_every=sort_every, inflow_every=4)
