@@ -1057,6 +1057,57 @@ __global__ void k_gather2(int f64, P64 Pd, P32 Qd, P64 Ps, P32 Qs,
   }
 }
 
+// the FP32 form's permutation gather, four slots per thread (the index as
+// one uint4, each array as one 16-byte store; the arrays start at multiples
+// of the capacity, itself a multiple of 256)
+template <typename T, typename V>
+__device__ inline void gather4_2(T *__restrict__ d, const T *__restrict__ s, const uint4 &j) {
+  V v;
+  v.x = s[j.x];
+  v.y = s[j.y];
+  v.z = s[j.z];
+  v.w = s[j.w];
+  *(V *)d = v;
+}
+
+__global__ void k_gather2_32(P32 d, P32 s, const uint32_t *__restrict__ idx, int64_t n) {
+  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= n) return;
+  if (i0 + 4 <= n) {
+    const uint4 j = *(const uint4 *)(idx + i0);
+    gather4_2<int32_t, int4>(d.cell + i0, s.cell, j);
+    gather4_2<float, float4>(d.ox + i0, s.ox, j);
+    gather4_2<float, float4>(d.oz + i0, s.oz, j);
+    gather4_2<float, float4>(d.vx + i0, s.vx, j);
+    gather4_2<float, float4>(d.vy + i0, s.vy, j);
+    gather4_2<float, float4>(d.vz + i0, s.vz, j);
+    return;
+  }
+  for (int64_t i = i0; i < n; ++i) {
+    const uint32_t j = idx[i];
+    d.cell[i] = s.cell[j];
+    d.ox[i] = s.ox[j];
+    d.oz[i] = s.oz[j];
+    d.vx[i] = s.vx[j];
+    d.vy[i] = s.vy[j];
+    d.vz[i] = s.vz[j];
+  }
+}
+
+// dst[i] = src[idx[i]] for i < n in the context's form
+static int gather_perm(pic2d_ctx *ctx, int sp, int src, int dst, int64_t n) {
+  if (n <= 0) return 0;
+  if (ctx->f64)
+    k_gather2<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->rt.stream>>>(
+        1, p64(ctx, sp, dst), p32(ctx, sp, dst), p64(ctx, sp, src), p32(ctx, sp, src),
+        ctx->rt.idx, n);
+  else
+    k_gather2_32<<<(unsigned)div_up(n, 4 * kBlock), kBlock, 0, ctx->rt.stream>>>(
+        p32(ctx, sp, dst), p32(ctx, sp, src), ctx->rt.idx, n);
+  P2_LAUNCH_CHECK(ctx);
+  return 0;
+}
+
 // stable compaction of species sp into its other buffer; the slots lose
 // their tile order
 static int compact(pic2d_ctx *ctx, int sp) {
@@ -1071,12 +1122,8 @@ static int compact(pic2d_ctx *ctx, int sp) {
   int64_t live = 0;
   int rc = pic::select_flagged(&ctx->rt, ctx->rt.flags, ctx->rt.idx, n, &live);
   if (rc) return rc;
-  if (live > 0) {
-    k_gather2<<<(unsigned)div_up(live, kBlock), kBlock, 0, ctx->rt.stream>>>(
-        f64, p64(ctx, sp, dst), p32(ctx, sp, dst), p64(ctx, sp, src),
-        p32(ctx, sp, src), ctx->rt.idx, live);
-    P2_LAUNCH_CHECK(ctx);
-  }
+  rc = gather_perm(ctx, sp, src, dst, live);
+  if (rc) return rc;
   ctx->cur[sp] = dst;
   ctx->n[sp] = live;
   return clear_tiles(ctx, sp);
@@ -1199,12 +1246,8 @@ int sort(pic2d_ctx *ctx, int sp) {
                              p32(ctx, sp, src).cell, *ctx->ts, ctx->rt.idx, &done, &valid);
     if (rc) return rc;
     if (done) {
-      if (valid > 0) {
-        k_gather2<<<(unsigned)div_up(valid, kBlock), kBlock, 0, st>>>(
-            f64, p64(ctx, sp, dst), p32(ctx, sp, dst), p64(ctx, sp, src),
-            p32(ctx, sp, src), ctx->rt.idx, valid);
-        P2_LAUNCH_CHECK(ctx);
-      }
+      rc = gather_perm(ctx, sp, src, dst, valid);
+      if (rc) return rc;
       ctx->cur[sp] = dst;
       ctx->n[sp] = valid;
       ctx->sorted_end[sp] = valid;
@@ -1230,12 +1273,8 @@ int sort(pic2d_ctx *ctx, int sp) {
                              sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   P2_HIP(ctx, hipStreamSynchronize(st));
   const int64_t valid = (int64_t)(*(uint32_t *)ctx->rt.host_counter);
-  if (valid > 0) {
-    k_gather2<<<(unsigned)div_up(valid, kBlock), kBlock, 0, st>>>(
-        f64, p64(ctx, sp, dst), p32(ctx, sp, dst), p64(ctx, sp, src),
-        p32(ctx, sp, src), ctx->rt.idx, valid);
-    P2_LAUNCH_CHECK(ctx);
-  }
+  rc = gather_perm(ctx, sp, src, dst, valid);
+  if (rc) return rc;
   ctx->cur[sp] = dst;
   ctx->n[sp] = valid;
   ctx->sorted_end[sp] = valid;
