@@ -102,6 +102,9 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None,
     if workload != "2d3v" and thrust:
         F, nsteps = sim.thrust_ave_get()
         thrust_sum = {"steps": nsteps, "F0_2": F[:3].tolist()}
+    # which sort ran: tile sorts and radix sorts over the context's life (the
+    # load's sort of the unsorted initial state is a radix sort)
+    st = sim.sort_stats()
     sim.close()
     per_launch_particles = particles / 2.0  # one launch per species
     push_avg_s = push_ms / push_n / 1e3
@@ -115,6 +118,7 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None,
                     else TRAFFIC_PER_PARTICLE)[form] * per_launch_particles,
         "bytes_per_particle": bpp,
         "thrust": thrust_sum,
+        "sorts": {"tile": st[0], "radix": st[1]},
     }
 
 
@@ -555,6 +559,7 @@ def main():
                      "frac_of_copy": main_r["achieved"] / copy_Bps if copy_Bps else None},
         "cpu_baseline": cpu,
         "kernel_ms_per_step": main_r["kernel_ms"],
+        "sorts": main_r["sorts"],
     }
     if ferr:
         out["field_error_vs_fp64"] = ferr

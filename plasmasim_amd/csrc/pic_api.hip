@@ -304,6 +304,7 @@ static int host_fields_b0(const pic_ctx *ctx, double *b0) {
 
 static void free_all(pic_ctx *ctx) {
   poisson_free(ctx);
+  f64::tsort_free(ctx);
   if (ctx->field_mem) (void)hipFree(ctx->field_mem);
   if (ctx->sig_mem) (void)hipFree(ctx->sig_mem);
   for (int s = 0; s < 2; ++s)
@@ -485,8 +486,8 @@ extern "C" int pic_create(const pic_config *cfg, pic_ctx **out) {
       return rc2;
     }
   }
-  if (!f64) {
-    int rc2 = f32::init_state(ctx);
+  {
+    int rc2 = f64 ? f64::tsort_setup(ctx) : f32::init_state(ctx);
     if (rc2) {
       fprintf(stderr, "pic_create: %s\n", ctx->err);
       free_all(ctx);

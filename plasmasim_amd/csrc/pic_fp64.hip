@@ -7,12 +7,14 @@
 // particles in SoA per species; a dead particle carries z = NaN until the
 // next compaction (inflow or sort).
 #include <hip/hip_runtime.h>
+#include <new>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "philox.h"
 #include "pic_internal.h"
+#include "pic_tsort.h"
 #include "pic_tiles.h"
 
 namespace pic {
@@ -1156,11 +1158,48 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   return 0;
 }
 
+// The tile sort's destination map of the FP64 form (pic_tsort.h): the
+// particles carry no cell index, so the sort runs on the key array of
+// k_sort_keys (tile * cells per tile + cell in the tile, or the drop key);
+// the home slot is the tile coordinates modulo 4, as in the FP32 form.
+struct KeyMapKey {
+  TileGeom tg;
+  uint32_t drop;  // tiles x cells per tile
+  __host__ __device__ uint32_t cells() const { return (uint32_t)(tg.TX * tg.TY * tg.TZ); }
+  __device__ bool dest(int32_t c, uint32_t &t, uint32_t &cin, int &h) const {
+    const uint32_t key = (uint32_t)c;
+    if (key >= drop) return false;
+    t = tg.tile_of(key);
+    cin = tg.cell_of(key);
+    const uint32_t tz = t % (uint32_t)tg.NTZ, r = t / (uint32_t)tg.NTZ;
+    const uint32_t ty = r % (uint32_t)tg.NTY, tx = r / (uint32_t)tg.NTY;
+    h = (int)(((tx & 3u) << 4) | ((ty & 3u) << 2) | (tz & 3u));
+    return true;
+  }
+};
+
+int tsort_setup(pic_ctx *ctx) {
+  const TileGeom tg = tile_geom(ctx);
+  // (slots are 32-bit in the tile sort: larger stores take the radix sort)
+  if (tg.ntiles() <= 1 || tg.TX * tg.TY * tg.TZ > tsort::kTileCells ||
+      ctx->cap > ((int64_t)1 << 30))
+    return 0;
+  ctx->ts64 = new (std::nothrow) tsort::Bufs();
+  if (!ctx->ts64) return set_error(ctx, PIC_E_NOMEM, "tile sort lists");
+  return tsort::reserve(ctx, *ctx->ts64, tg.ntiles(), ctx->cap);
+}
+
+void tsort_free(pic_ctx *ctx) {
+  if (!ctx->ts64) return;
+  tsort::release(*ctx->ts64);
+  delete ctx->ts64;
+  ctx->ts64 = nullptr;
+}
+
 int sort(pic_ctx *ctx, int sp) {
   const Geom &g = ctx->geom;
   const int64_t n = ctx->n[sp];
   if (n == 0) return 0;
-  ctx->sort_radix_count++;
   Particles64 src = p64(ctx, sp, ctx->cur[sp]);
   const TileGeom tg = tile_geom(ctx);
   const int nt = tg.ntiles();
@@ -1168,6 +1207,28 @@ int sort(pic_ctx *ctx, int sp) {
       g, tg, src, n, ctx->cfg.dx, ctx->keys);
   PIC_LAUNCH_CHECK(ctx);
   const uint32_t ncell = (uint32_t)g.NX * g.NY * g.NZ;
+  if (ctx->ts64 && ctx->sorted_end[sp] > 0 && ctx->sorted_end[sp] <= n) {
+    // the tiles of the last sort are current: the tile sort of the keys
+    const KeyMapKey km{tg, (uint32_t)nt * (uint32_t)(tg.TX * tg.TY * tg.TZ)};
+    bool done = false;
+    int64_t valid = 0;
+    int rc = tsort::run(ctx, km, nt, ctx->tile_start[sp], ctx->sorted_end[sp], n,
+                        (const int32_t *)ctx->keys, *ctx->ts64, ctx->idx, &done, &valid);
+    if (rc) return rc;
+    if (done) {
+      Particles64 dst = p64(ctx, sp, 1 - ctx->cur[sp]);
+      if (valid > 0)
+        k_gather_particles<<<(unsigned)div_up(valid, kBlock), kBlock, 0, ctx->stream>>>(
+            dst, src, ctx->idx, valid);
+      PIC_LAUNCH_CHECK(ctx);
+      ctx->cur[sp] = 1 - ctx->cur[sp];
+      ctx->n[sp] = valid;
+      ctx->sorted_end[sp] = valid;
+      ctx->sort_tile_count++;
+      return 0;
+    }
+  }
+  ctx->sort_radix_count++;
   int bits = 1;
   while (bits < 32 && ((uint64_t)1 << bits) <= ncell) ++bits;
   int rc = sort_keys(ctx, ctx->keys, ctx->keys_alt, ctx->idx, n, bits);

@@ -90,6 +90,12 @@ struct StepConst {
 
 }  // namespace pic
 
+namespace pic {
+namespace tsort {
+struct Bufs;
+}
+}  // namespace pic
+
 struct pic_ctx {
   pic_config cfg;
   pic::Geom geom;
@@ -141,7 +147,8 @@ struct pic_ctx {
   uint32_t *tile_start[2] = {nullptr, nullptr};   // ntiles + 1 per species
   int64_t sorted_end[2] = {0, 0};
   int32_t drift_halves[2] = {0, 0};  // half-step drifts since the tile sort
-  int64_t sort_tile_count = 0;   // sorts done by the tile sort (FP32 form)
+  int64_t sort_tile_count = 0;   // sorts done by the tile sort
+  pic::tsort::Bufs *ts64 = nullptr;  // the FP64 form's tile-sort lists (pic_tsort.h)
   int64_t sort_radix_count = 0;  // sorts done by the radix sort
 
   // state private to the numeric form (FP32: sigma copy, tile ranges,
@@ -261,6 +268,8 @@ int push(pic_ctx *, int sp, int do_push_j, int do_drift_n, int do_boundary);
 int particle_boundary(pic_ctx *, int sp);
 int inflow(pic_ctx *, int sp, uint32_t event);
 int sort(pic_ctx *, int sp);
+int tsort_setup(pic_ctx *);  // the tile sort's lists, at create
+void tsort_free(pic_ctx *);
 int diagnose(pic_ctx *, pic_diag *);
 int count_live(pic_ctx *, int sp, int64_t *n);
 }  // namespace f64

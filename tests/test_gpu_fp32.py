@@ -67,13 +67,15 @@ def test_tile_sort_permutation_matches_oracle():
 
 
 @pytest.mark.parametrize("tile", [(4, 4, 8), (8, 4, 4), (2, 2, 2)])
-def test_tile_sort_after_steps_matches_oracle(tile):
+@pytest.mark.parametrize("precision", [ps.PIC_FP32, ps.PIC_FP64])
+def test_tile_sort_after_steps_matches_oracle(tile, precision):
     """The tile sort meets what a periodic sort meets: particles tile-sorted
     by the last sort, most still in their tile, some moved to other tiles,
     dead slots and an inflow tail.  The live particles it leaves are the
     oracle's stable sort of the same state, bit for bit and in order, and
-    the sort took the tile path (not the radix sort)."""
-    c = cfg(1, tile=tile)
+    the sort took the tile path (not the radix sort).  The FP64 form runs
+    the same passes on its key array."""
+    c = cfg(1, precision=precision, tile=tile)
     c.sort_every, c.inflow_every = 1000, 5
     g = ps.Simulation(c)
     g.load_particles()
