@@ -2,7 +2,7 @@
 // translation units.  Not part of the C-ABI.
 //
 // The context embeds a pic_ctx as its runtime: the HIP stream, the error
-// text, the compaction/sort scratch, the rocPRIM temporary storage and the
+// text, the compaction/sort scratch, the sort's temporary storage and the
 // timing events are the 3D context's, so the device-wide building blocks
 // of pic_prim.hip serve both.
 #pragma once

@@ -119,7 +119,7 @@ struct pic_ctx {
   uint32_t *keys = nullptr, *keys_alt = nullptr;
   uint32_t *idx = nullptr, *idx_alt = nullptr;
   uint8_t *flags = nullptr;
-  void *prim_tmp = nullptr;        // rocPRIM temporary storage
+  void *prim_tmp = nullptr;        // radix sort and selection temporaries
   size_t prim_tmp_bytes = 0;
   int64_t *dev_counter = nullptr;  // 8 int64 device scratch slots
   int64_t *host_counter = nullptr; // pinned mirror
@@ -332,7 +332,7 @@ int push_fused(pic_ctx *ctx, int sp);
 // adjoint state (pic_adj.hip)
 void adj_free(pic_ctx *ctx);
 
-// device-wide building blocks on rocPRIM (pic_prim.hip)
+// device-wide building blocks, hand-written (pic_prim.hip)
 // stable selection: idx_out[0..*count) = indices i < n with flags[i] != 0
 int select_flagged(pic_ctx *ctx, const uint8_t *flags, uint32_t *idx_out,
                    int64_t n, int64_t *count);
