@@ -100,6 +100,42 @@ def test_tile_sort_after_steps_matches_oracle(tile, precision):
     g.close()
 
 
+@pytest.mark.parametrize("precision", [ps.PIC_FP32, ps.PIC_FP64])
+def test_tile_sort_tiles_not_powers_of_two(precision):
+    """Tiles of 3 x 4 x 6 cells on a 24^3 grid: the tile keys, the tile
+    sort's destinations and the FP64 form's key decoding take their division
+    path (the shifts serve power-of-two tiles only).  After 12 steps the sort
+    takes the tile path, equals the oracle's stable sort of the same
+    particles, and the steps after it run on the new ranges."""
+    def conf(prec):
+        c = ps.preset(1)
+        c.precision = prec
+        c.nx = c.ny = c.nz = 24
+        c.tile[0], c.tile[1], c.tile[2] = 3, 4, 6
+        c.sort_every, c.inflow_every = 1000, 5
+        return c
+    g = ps.Simulation(conf(precision))
+    g.load_particles()
+    g.begin()
+    g.step(12)
+    pre = [g.download_particles(sp) for sp in (0, 1)]
+    t0, r0 = g.sort_stats()
+    g.sort(0)
+    g.sort(1)
+    t1, r1 = g.sort_stats()
+    assert (t1 - t0, r1 - r0) == (2, 0)
+    o = Oracle(conf(ps.PIC_FP64))
+    for sp in (0, 1):
+        o.upload_particles(sp, pre[sp])
+        o.sort(sp)
+        a, b = g.download_particles(sp), o.download_particles(sp)
+        assert a.shape == b.shape and len(a) > 0
+        assert np.array_equal(a, b), sp
+    g.step(3)
+    assert g.particle_count(0) > 0 and g.particle_count(1) > 0
+    g.close()
+
+
 def cluster(seed, vel):
     """20000 electrons in one 2 x 2 x 2 tile of the ballistic layer (drift
     only), velocities from vel(rng, n)."""
