@@ -538,16 +538,17 @@ __device__ inline void glb_add32(float *p, float v) {
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// one lane's stencil straight into a global plane
-__device__ __noinline__ void glob_dep4(float *glob, const Geom2 &g, int k, int m,
-                                       float q, float a0, float a1, float c0,
-                                       float c1) {
-  const int64_t i = g.idx(k, m);
+// one lane's stencil straight into a global plane of row stride SZ (taken by
+// value: a reference to the kernel's geometry argument from this out-of-line
+// function would keep a copy of it in private memory)
+__device__ __noinline__ void glob_dep4(float *glob, int SZ, int k, int m, float q,
+                                       float a0, float a1, float c0, float c1) {
+  const int64_t i = (int64_t)k * SZ + m;
   const float qa0 = q * a0, qa1 = q * a1;
   glb_add32(glob + i, qa0 * c0);
-  glb_add32(glob + i + g.SZ, qa1 * c0);
+  glb_add32(glob + i + SZ, qa1 * c0);
   glb_add32(glob + i + 1, qa0 * c1);
-  glb_add32(glob + i + g.SZ + 1, qa1 * c1);
+  glb_add32(glob + i + SZ + 1, qa1 * c1);
 }
 
 constexpr int kCoopMax = 8;
@@ -565,11 +566,11 @@ __device__ __forceinline__ void glob_coop2(const Geom2 &g, const Dep32 &D, bool 
     if (pend) {
       const Ax X = axis2(k, ox), Z = axis2(m, oz);
       if (current) {
-        glob_dep4(D.jx, g, X.bi, Z.bh, q0, X.i0, X.i1, Z.h0, Z.h1);
-        glob_dep4(D.jy, g, X.bh, Z.bh, q1, X.h0, X.h1, Z.h0, Z.h1);
-        glob_dep4(D.jz, g, X.bh, Z.bi, q2, X.h0, X.h1, Z.i0, Z.i1);
+        glob_dep4(D.jx, g.SZ, X.bi, Z.bh, q0, X.i0, X.i1, Z.h0, Z.h1);
+        glob_dep4(D.jy, g.SZ, X.bh, Z.bh, q1, X.h0, X.h1, Z.h0, Z.h1);
+        glob_dep4(D.jz, g.SZ, X.bh, Z.bi, q2, X.h0, X.h1, Z.i0, Z.i1);
       } else {
-        glob_dep4(D.n, g, X.bh, Z.bh, q0, X.h0, X.h1, Z.h0, Z.h1);
+        glob_dep4(D.n, g.SZ, X.bh, Z.bh, q0, X.h0, X.h1, Z.h0, Z.h1);
       }
     }
     return;

@@ -56,14 +56,22 @@ struct Stn {
   double w[3][2];
 };
 
-__device__ inline void stencil(double dx, const double p[3], int c, Stn &s) {
+// q = p / dx, computed once per position by callers that build several
+// stencils at it
+__device__ inline void stencil_q(double dx, const double p[3], const double q[3], int c,
+                                 Stn &s) {
   for (int a = 0; a < 3; ++a) {
     const double h = kStg[c][a] ? 0.5 : 0.0;
-    const int k = (int)(p[a] / dx - h);
+    const int k = (int)(q[a] - h);
     s.b[a] = k;
     s.w[a][0] = (k + 1 + h) * dx - p[a];
     s.w[a][1] = p[a] - (k + h) * dx;
   }
+}
+
+__device__ inline void stencil(double dx, const double p[3], int c, Stn &s) {
+  const double q[3] = {p[0] / dx, p[1] / dx, p[2] / dx};
+  stencil_q(dx, p, q, c, s);
 }
 
 // value and position gradient of sum_n F[n] W_n (W unnormalised)
@@ -104,10 +112,9 @@ __device__ inline void st_scatter(const Geom &g, double *F, const Stn &s, double
   }
 }
 
-__device__ inline bool in_array(const Geom &g, const double p[3], double dx) {
-  const double a = p[0] / dx, b = p[1] / dx, c = p[2] / dx;
-  return a > -0.5 && a < g.SX - 1 && b > -0.5 && b < g.SY - 1 && c > -0.5 &&
-         c < g.SZ - 1;
+__device__ inline bool in_array_q(const Geom &g, const double q[3]) {
+  return q[0] > -0.5 && q[0] < g.SX - 1 && q[1] > -0.5 && q[1] < g.SY - 1 &&
+         q[2] > -0.5 && q[2] < g.SZ - 1;
 }
 
 __device__ inline double sigma_at(const SigmaProfile &s, int k2, int l2, int m2) {
@@ -486,7 +493,7 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
   const int tid = threadIdx.x, lane = tid & 63;
   const double dx = K.dx, dth = K.dth;
   bool live = i < n && !isnan(P.z[i]);
-  double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0};
+  double x0[3] = {0, 0, 0}, v0[3] = {0, 0, 0}, q0[3] = {0, 0, 0};
   bool pushed = false;
   if (live) {
     x0[0] = P.x[i];
@@ -497,14 +504,15 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
     v0[2] = P.vz[i];
     const bool ball = x0[0] < K.blo[0] || x0[0] > K.bhi[0] || x0[1] < K.blo[1] ||
                       x0[1] > K.bhi[1] || x0[2] < K.blo[2] || x0[2] > K.bhi[2];
-    pushed = !ball && in_array(g, x0, dx);
+    for (int a = 0; a < 3; ++a) q0[a] = x0[a] / dx;
+    pushed = !ball && in_array_q(g, q0);
   }
   // the nodes the gathers at x0 touch: from the half-staggered base (int)(x -
   // 1/2) to the integer base (int)(x) + 1 on each axis
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
-    lo[a] = pushed ? (int)(x0[a] / dx - 0.5) : (1 << 29);
-    hi[a] = pushed ? (int)(x0[a] / dx) + 1 : -(1 << 29);
+    lo[a] = pushed ? (int)(q0[a] - 0.5) : (1 << 29);
+    hi[a] = pushed ? (int)q0[a] + 1 : -(1 << 29);
   }
   for (int o = 32; o > 0; o >>= 1)
     for (int a = 0; a < 3; ++a) {
@@ -534,17 +542,21 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
     if (pushed) {
       for (int c = 0; c < 6; ++c) {
         Stn s;
-        stencil(dx, x0, c, s);
+        stencil_q(dx, x0, q0, c, s);
         f[c] = st_value(g, F.f[c], s) * K.inv3;
       }
       boris_fwd(K.bk, f, v0, vp);
     } else {
       for (int a = 0; a < 3; ++a) vp[a] = v0[a];
     }
-    for (int a = 0; a < 3; ++a) x1[a] = x0[a] + vp[a] * dth;
+    double q1[3];
+    for (int a = 0; a < 3; ++a) {
+      x1[a] = x0[a] + vp[a] * dth;
+      q1[a] = x1[a] / dx;
+    }
     const bool dep = x1[0] < (g.NX + 4) * dx && x1[0] >= 0 && x1[1] < (g.NY + 4) * dx &&
                      x1[1] >= 0 && x1[2] < (g.NZ + 4) * dx && x1[2] >= 0 &&
-                     in_array(g, x1, dx);
+                     in_array_q(g, q1);
     double *lpi = lp + 6 * i;
     double lx[3] = {lpi[0], lpi[1], lpi[2]};
     double lvp[3] = {lpi[3], lpi[4], lpi[5]};
@@ -553,7 +565,7 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
       for (int c = 0; c < 3; ++c) {
         Stn s;
         double gr[3];
-        stencil(dx, x1, c, s);
+        stencil_q(dx, x1, q1, c, s);
         const double val = st_gather(g, L.J[sp][c], s, gr);
         lvp[c] += K.qn * val;
         for (int a = 0; a < 3; ++a) lx[a] += K.qn * vp[c] * gr[a];
@@ -567,7 +579,7 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
       for (int c = 0; c < 6; ++c) {
         Stn s;
         double gr[3];
-        stencil(dx, x0, c, s);
+        stencil_q(dx, x0, q0, c, s);
         st_gather(g, F.f[c], s, gr);
         const double w = lf[c] * K.inv3;
         for (int a = 0; a < 3; ++a) lx[a] += w * gr[a];
@@ -582,8 +594,12 @@ __global__ __launch_bounds__(256) void k_pass_T(Geom g, PD P, int64_t n, F6 F,
             const int nn = t ^ (lane & 7);
             const int a = nn >> 2, b2 = (nn >> 1) & 1, c2 = nn & 1;
             const int j = ((bi + a) * WY + (bj + b2)) * WZ + (bk + c2);
-            __hip_atomic_fetch_add((PIC_LDS double *)&win[c * kAdjWin + j],
-                                   w * s.w[0][a] * s.w[1][b2] * s.w[2][c2],
+            // selects, not s.w[.][lane-dependent index]: that form puts the
+            // stencil in private memory
+            const double wa = a ? s.w[0][1] : s.w[0][0];
+            const double wb = b2 ? s.w[1][1] : s.w[1][0];
+            const double wc = c2 ? s.w[2][1] : s.w[2][0];
+            __hip_atomic_fetch_add((PIC_LDS double *)&win[c * kAdjWin + j], w * wa * wb * wc,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         } else {
