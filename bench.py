@@ -96,7 +96,8 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None,
     rate = particles * steps / (t1 - t0)
     tsteps = warmup + steps
     push_ms, push_n = sim.timing_get("push")
-    tot = {k: sim.timing_get(k) for k in ("push", "field_b", "field_e", "sort", "inflow", "other")}
+    tot = {k: sim.timing_get(k)
+           for k in ("push", "push_sort", "field_b", "field_e", "sort", "inflow", "other")}
     sim.timing(False)
     thrust_sum = None
     if workload != "2d3v" and thrust:

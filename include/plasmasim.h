@@ -230,7 +230,10 @@ int pic_diagnose_device(pic_ctx *ctx, pic_diag *dev_out);
 
 /* fdtd() step loop (:687-949): nsteps full steps with the configured sort
  * and inflow cadences, fusing loops A and B and the particle boundary into
- * one particle pass per species. */
+ * one particle pass per species.  FP32 form: a step's tile sort copies no
+ * particles, the species' pass of that step reads them through the sort's
+ * permutation; on a step with both a sort and an inflow the inflow runs
+ * first (the same particles as the reference's order, in other slots). */
 int pic_step(pic_ctx *ctx, int nsteps);
 /* thrust_fave (:926, :5124-5285), every step of pic_step: the Lorentz force
  * q (E + v x B) n dx^3 on the particles of the central box (0.2 N dx <
@@ -276,7 +279,9 @@ int pic_set_counters(pic_ctx *ctx, int64_t step, int32_t c_add, int32_t c_sort);
 #define PIC_T_SORT 3
 #define PIC_T_INFLOW 4
 #define PIC_T_OTHER 5    /* halos, fold, zeroing */
-#define PIC_T_COUNT 6
+#define PIC_T_PUSH_SORT 6 /* fused particle pass applying pic_step's sort
+                             permutation (FP32 form) */
+#define PIC_T_COUNT 7
 int pic_timing_enable(pic_ctx *ctx, int on);
 /* Total milliseconds and launch count accumulated for a timing class. */
 int pic_timing_get(pic_ctx *ctx, int which, double *ms, int64_t *launches);

@@ -23,7 +23,7 @@ FIELDS = ["ex", "ey", "ez", "bx", "by", "bz", "b0x", "b0y", "b0z",
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 FIELD_ID["phi"] = len(FIELDS)  # PIC_PHI: download only
 TIMING = {"push": 0, "field_b": 1, "field_e": 2, "sort": 3, "inflow": 4,
-          "other": 5}
+          "other": 5, "push_sort": 6}
 
 
 class PicConfig(ctypes.Structure):

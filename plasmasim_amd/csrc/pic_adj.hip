@@ -1224,7 +1224,9 @@ extern "C" int pic_adj_forward(pic_ctx *ctx, const double params[4], int nsteps,
   if (radius > cmin - 2)
     return set_error(ctx, PIC_E_ARG, "adjoint: sphere radius %g > %d cells", radius,
                      cmin - 2);
-  int rc = ensure_state(ctx);
+  int rc = ctx->cfg.precision == PIC_FP64 ? 0 : f32::settle_sorts(ctx);
+  if (rc) return rc;
+  rc = ensure_state(ctx);
   if (rc) return rc;
   AdjState *A = state(ctx);
   free_ckpts(A);

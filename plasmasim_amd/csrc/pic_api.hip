@@ -986,25 +986,47 @@ int push_fused(pic_ctx *ctx, int sp) {
 extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
   if (!ctx || nsteps < 0) return PIC_E_ARG;
   const pic_config &c = ctx->cfg;
+  const bool f64 = is_f64(ctx);
   int rc;
-#define TRY(x)              \
-  do {                      \
-    if ((rc = (x))) return rc; \
+  // an error leaves no sort permutation waiting for a pass
+#define TRY(x)                                  \
+  do {                                          \
+    if ((rc = (x))) {                           \
+      if (!f64) (void)f32::settle_sorts(ctx);   \
+      return rc;                                \
+    }                                           \
   } while (0)
+  // The FP32 form's sort leaves its permutation to the step's fused pass,
+  // which reads each particle through it (sort_for_pass): no copy of the
+  // particles.  When the sort and the inflow fall on the same step the
+  // inflow runs first, so that nothing but field work lies between the sort
+  // and the pass.  The inflow draws each particle from the counter of its
+  // reference-order index and the sort only permutes, so either order leaves
+  // the same particles; only their slots differ.
+  const auto sort_both = [&]() -> int {
+    ctx->c_sort = 0;
+    for (int sp = 0; sp < 2; ++sp) {
+      timing_begin(ctx);
+      const int r = f64 ? f64::sort(ctx, sp) : f32::sort_for_pass(ctx, sp);
+      timing_end(ctx, PIC_T_SORT);
+      if (r) return r;
+    }
+    return 0;
+  };
   for (int s = 0; s < nsteps; ++s) {
     TRY(pic_renew_grid(ctx));
-    if (ctx->c_sort > c.sort_every - 1) {
-      ctx->c_sort = 0;
-      TRY(pic_sort(ctx, PIC_ION));
-      TRY(pic_sort(ctx, PIC_ELECTRON));
-    }
-    if (ctx->c_add > c.inflow_every - 1) {
+    const bool sort_due = ctx->c_sort > c.sort_every - 1;
+    const bool add_due = ctx->c_add > c.inflow_every - 1;
+    const bool inflow_first = !f64 && sort_due && add_due;
+    if (sort_due && !inflow_first) TRY(sort_both());
+    if (add_due) {
       // the inflow removes z < 4 dx (:6019-6021); where thrust_fave's box
       // reaches below that (nz < 20) the pending force is taken first
       if (ctx->thrust_pending && c.nz * 0.2 < 4.0) TRY(thrust_settle(ctx));
       TRY(pic_inflow(ctx));
       ctx->c_add = 0;
     }
+    if (inflow_first) TRY(sort_both());
     if (ctx->thrust_pending) {
       TRY(thrust_keep_b(ctx));
       ctx->thrust_now = 1;
@@ -1012,11 +1034,13 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
     TRY(pic_renew_b_half(ctx));
     TRY(pic_halo_b(ctx));
     for (int sp = 0; sp < 2; ++sp) {
+      const int tclass = !f64 && f32::pending(ctx, sp) ? PIC_T_PUSH_SORT : PIC_T_PUSH;
       timing_begin(ctx);
       rc = DISPATCH(push, ctx, sp, 1, 1, 1);
-      timing_end(ctx, PIC_T_PUSH);
+      timing_end(ctx, tclass);
       if (rc) {
         ctx->thrust_now = 0;
+        if (!f64) (void)f32::settle_sorts(ctx);
         return rc;
       }
     }

@@ -190,6 +190,7 @@ extern "C" int pic_checkpoint_save(pic_ctx *ctx, const char *path) {
   const bool f64 = ctx->cfg.precision == PIC_FP64;
   if (!f64) {
     int rc = f32::finish_deposit(ctx);
+    if (!rc) rc = f32::settle_sorts(ctx);
     if (rc) return rc;
   }
   PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -259,6 +260,10 @@ extern "C" int pic_checkpoint_save(pic_ctx *ctx, const char *path) {
 extern "C" int pic_checkpoint_load(pic_ctx *ctx, const char *path) {
   if (!ctx || !path) return PIC_E_ARG;
   const bool f64 = ctx->cfg.precision == PIC_FP64;
+  if (!f64) {
+    int rc = f32::settle_sorts(ctx);  // the load replaces the particle state
+    if (rc) return rc;
+  }
   PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const Geom &g = ctx->geom;
   const size_t nc = (size_t)g.SX * g.SY * g.SZ;

@@ -594,11 +594,15 @@ __device__ inline void thrust_store(const float (&th)[6], double *out, int bid) 
   }
 }
 
-template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, bool THRUST>
+// PERM: the pass also applies a sort's permutation: slot i takes its
+// particle from slot perm[i] of the arrays Q and stores the result in P (the
+// other buffer of the species), so the sort needs no separate copy
+template <bool PUSH_J, bool DRIFT_N, bool BOUNDARY, bool DET, bool THRUST, bool PERM>
 __global__ __launch_bounds__(kPassBlock) PIC_WAVES32 void k_particles32(
     Grid32 G, TileGeom tg, Particles32 P, TileRun tr, Push32 kp,
     typename GlobT<DET>::T *jx, typename GlobT<DET>::T *jy,
-    typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd, double *thrust_part) {
+    typename GlobT<DET>::T *jz, typename GlobT<DET>::T *nd, double *thrust_part,
+    Particles32 Q, const uint32_t *__restrict__ perm) {
   using A = typename GlobT<DET>::T;
   __shared__ u64 acc_mem[4 * kWin];
   __shared__ Win sw;
@@ -613,7 +617,7 @@ __global__ __launch_bounds__(kPassBlock) PIC_WAVES32 void k_particles32(
     Win w;
     tile_setup(bid, tr, tg, kWin,
                [&](int64_t i, int &k, int &l, int &m) {
-                 const int32_t cl = P.cell[i];
+                 const int32_t cl = PERM ? Q.cell[perm[i]] : P.cell[i];
                  if (cl < 0) return false;
                  unpack(cl, k, l, m);
                  return true;
@@ -698,13 +702,34 @@ __global__ __launch_bounds__(kPassBlock) PIC_WAVES32 void k_particles32(
   for (int q = q0 + tid; q < q1; q += blockDim.x) {
     const int i0 = q << SH;
     const uint32_t bo = (uint32_t)i0 * 4u;
-    vi cs = *(const vi *)vat(P.cell, bo);
-    vf xs = *(const vf *)vat(P.ox, bo);
-    vf ys = *(const vf *)vat(P.oy, bo);
-    vf zs = *(const vf *)vat(P.oz, bo);
-    vf us = *(const vf *)vat(P.vx, bo);
-    vf vs = *(const vf *)vat(P.vy, bo);
-    vf ws = *(const vf *)vat(P.vz, bo);
+    vi cs;
+    vf xs, ys, zs, us, vs, ws;
+    if (PERM) {
+      // the capacity is a multiple of 256, so the group's perm entries are
+      // allocated; slots outside the range read slot 0 and are not stored
+      typedef unsigned vu __attribute__((ext_vector_type(PIC_PPT32)));
+      const vu pi = *(const vu *)vat(perm, bo);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const int i = i0 + j;
+        const uint32_t sb = i >= s32 && i < e32 ? pi[j] * 4u : 0u;
+        cs[j] = *vat(Q.cell, sb);
+        xs[j] = *vat(Q.ox, sb);
+        ys[j] = *vat(Q.oy, sb);
+        zs[j] = *vat(Q.oz, sb);
+        us[j] = *vat(Q.vx, sb);
+        vs[j] = *vat(Q.vy, sb);
+        ws[j] = *vat(Q.vz, sb);
+      }
+    } else {
+      cs = *(const vi *)vat(P.cell, bo);
+      xs = *(const vf *)vat(P.ox, bo);
+      ys = *(const vf *)vat(P.oy, bo);
+      zs = *(const vf *)vat(P.oz, bo);
+      us = *(const vf *)vat(P.vx, bo);
+      vs = *(const vf *)vat(P.vy, bo);
+      ws = *(const vf *)vat(P.vz, bo);
+    }
     const bool full = i0 >= s32 && i0 + N - 1 < e32;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -1452,9 +1477,19 @@ struct State32 {
   bool fx_dirty = false;
   double fx_scale = 0;
   tsort::Bufs ts;  // the tile sort's lists (mover pool of capacity entries)
+  // a tile sort of pic_step whose permutation the species' next fused pass
+  // applies (set only between the sort and that pass of one step)
+  bool pending[2] = {false, false};
 };
 
 static State32 *st(pic_ctx *ctx) { return (State32 *)ctx->form_state; }
+
+// where a deferred permutation waits for its pass: the ions' in idx_alt,
+// the electrons' in idx (the electron sort and everything up to the passes
+// leave idx_alt alone)
+static uint32_t *pass_perm(pic_ctx *ctx, int sp) {
+  return sp == PIC_ION ? ctx->idx_alt : ctx->idx;
+}
 
 static Sigma32 sigma_f(pic_ctx *ctx) {
   Sigma32 s;
@@ -1642,6 +1677,10 @@ int fold_current(pic_ctx *ctx) {
 }
 
 int load_particles(pic_ctx *ctx) {
+  {
+    int rc = settle_sorts(ctx);
+    if (rc) return rc;
+  }
   const pic_config &c = ctx->cfg;
   const int64_t n = (int64_t)c.nx * c.ny * c.nz * c.ppc;
   if (n > ctx->cap)
@@ -1705,6 +1744,12 @@ static Push32 push_const(const pic_ctx *ctx, int sp) {
 
 int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
          int do_boundary) {
+  // a pending sort permutation is applied by the fused pass only
+  const bool fused = do_push_j && do_drift_n && do_boundary;
+  if (!fused) {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const int64_t n = ctx->n[sp];
   if (n == 0) return 0;
   // slot indices and byte offsets of the pass are 32-bit
@@ -1724,8 +1769,12 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
   tr.n = n;
   tr.ntail = (int)div_up(n - tr.sorted_end, kTailChunk);
   const unsigned nb = (unsigned)(tr.ntiles + tr.ntail);
-  if (nb == 0) return 0;
-  Particles32 P = p32(ctx, sp, ctx->cur[sp]);
+  if (nb == 0) return settle(ctx, sp);
+  // with a pending permutation the pass reads the current buffer through it
+  // and writes the other one
+  const uint32_t *pm = S->pending[sp] ? pass_perm(ctx, sp) : nullptr;
+  Particles32 Q = p32(ctx, sp, ctx->cur[sp]);
+  Particles32 P = pm ? p32(ctx, sp, 1 - ctx->cur[sp]) : Q;
   Push32 kp = push_const(ctx, sp);
   Grid32 G = grid32(ctx);
   const bool det = S->fx != nullptr;
@@ -1764,11 +1813,15 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
                        *jz = base + (off + 2) * pl,
                        *nd = base + (6 + (sp == PIC_ION ? 0 : 1)) * pl;
 #define L(A, B, C) \
-  k_particles32<A, B, C, true, false><<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
+  k_particles32<A, B, C, true, false, false><<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp, Q, pm)
     switch (sel == 7 ? 8 : sel) {
       case 8:
-        k_particles32<true, true, true, true, true>
-            <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
+        if (pm)
+          k_particles32<true, true, true, true, true, true>
+              <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp, Q, pm);
+        else
+          k_particles32<true, true, true, true, true, false>
+              <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp, Q, pm);
         break;
       case 6: L(true, true, false); break;
       case 4: L(true, false, false); break;
@@ -1787,11 +1840,15 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
     float *jz = F.f[sp == PIC_ION ? PIC_JIZ : PIC_JEZ];
     float *nd = F.f[sp == PIC_ION ? PIC_NI : PIC_NE];
 #define L(A, B, C) \
-  k_particles32<A, B, C, false, false><<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp)
+  k_particles32<A, B, C, false, false, false><<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp, Q, pm)
     switch (sel == 7 ? 8 : sel) {
       case 8:
-        k_particles32<true, true, true, false, true>
-            <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp);
+        if (pm)
+          k_particles32<true, true, true, false, true, true>
+              <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp, Q, pm);
+        else
+          k_particles32<true, true, true, false, true, false>
+              <<<nb, kPassBlock, 0, ctx->stream>>>(G, tg, P, tr, kp, jx, jy, jz, nd, tp, Q, pm);
         break;
       case 6: L(true, true, false); break;
       case 4: L(true, false, false); break;
@@ -1804,6 +1861,10 @@ int push(pic_ctx *ctx, int sp, int do_push_j, int do_drift_n,
 #undef L
   }
   PIC_LAUNCH_CHECK(ctx);
+  if (pm) {
+    ctx->cur[sp] = 1 - ctx->cur[sp];
+    S->pending[sp] = false;
+  }
   if (thrust) {
     const pic_config &c = ctx->cfg;
     const double qs = sp == PIC_ION ? c.q : -c.q;
@@ -1840,6 +1901,10 @@ static int compact(pic_ctx *ctx, int sp) {
 // add_*_flat: slab particles become dead slots in place, new particles are
 // appended after the stored ones (the sort compacts)
 int inflow(pic_ctx *ctx, int sp, uint32_t event) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const pic_config &c = ctx->cfg;
   const Geom &g = ctx->geom;
   if (ctx->n[sp] > 0) {
@@ -1910,9 +1975,42 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
   return 0;
 }
 
-int sort(pic_ctx *ctx, int sp) {
+// apply a deferred sort permutation with the permutation gather (an error
+// path of pic_step, or a call other than the fused pass)
+int settle(pic_ctx *ctx, int sp) {
+  State32 *S = st(ctx);
+  if (!S || !S->pending[sp]) return 0;
+  S->pending[sp] = false;
+  const int64_t n = ctx->n[sp];
+  Particles32 src = p32(ctx, sp, ctx->cur[sp]);
+  Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
+  k_gather32<<<(unsigned)div_up(n, 4 * kBlock), kBlock, 0, ctx->stream>>>(
+      dst, src, pass_perm(ctx, sp), n);
+  PIC_LAUNCH_CHECK(ctx);
+  ctx->cur[sp] = 1 - ctx->cur[sp];
+  return 0;
+}
+
+bool pending(const pic_ctx *ctx, int sp) {
+  const State32 *S = (const State32 *)ctx->form_state;
+  return S && S->pending[sp];
+}
+
+int settle_sorts(pic_ctx *ctx) {
+  int rc = settle(ctx, PIC_ION);
+  const int re = settle(ctx, PIC_ELECTRON);
+  return rc ? rc : re;
+}
+
+// defer: a tile sort leaves its permutation for the species' next fused
+// pass instead of copying the particles (pic_step, FP32 form)
+static int sort_impl(pic_ctx *ctx, int sp, bool defer) {
   const Geom &g = ctx->geom;
   State32 *S = st(ctx);
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   ctx->drift_halves[sp] = 0;
   const TileGeom tg = tile_geom(ctx);
   const int nt = tg.ntiles();
@@ -1927,16 +2025,24 @@ int sort(pic_ctx *ctx, int sp) {
     bool done = false;
     int64_t valid = 0;
     const KeyMap3 km{g, tg};
+    uint32_t *perm = defer ? pass_perm(ctx, sp) : ctx->idx;
     int rc = tsort::reserve(ctx, S->ts, nt, ctx->cap);
     if (!rc)
       rc = tsort::run(ctx, km, nt, ctx->tile_start[sp], ctx->sorted_end[sp], n, src.cell,
-                      S->ts, ctx->idx, &done, &valid);
+                      S->ts, perm, &done, &valid);
     if (rc) return rc;
+    if (done && defer && valid > 0) {
+      ctx->n[sp] = valid;
+      ctx->sorted_end[sp] = valid;
+      ctx->sort_tile_count++;
+      S->pending[sp] = true;
+      return 0;
+    }
     if (done) {
       Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
       if (valid > 0)
         k_gather32<<<(unsigned)div_up(valid, 4 * kBlock), kBlock, 0, ctx->stream>>>(
-            dst, src, ctx->idx, valid);
+            dst, src, perm, valid);
       PIC_LAUNCH_CHECK(ctx);
       ctx->cur[sp] = 1 - ctx->cur[sp];
       ctx->n[sp] = valid;
@@ -1974,7 +2080,15 @@ int sort(pic_ctx *ctx, int sp) {
   return 0;
 }
 
+int sort(pic_ctx *ctx, int sp) { return sort_impl(ctx, sp, false); }
+
+int sort_for_pass(pic_ctx *ctx, int sp) { return sort_impl(ctx, sp, true); }
+
 int count_live(pic_ctx *ctx, int sp, int64_t *out) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const int64_t n = ctx->n[sp];
   *out = 0;
   if (n == 0) return 0;
@@ -1994,6 +2108,10 @@ int count_live(pic_ctx *ctx, int sp, int64_t *out) {
 static_assert(kNQF == kDiagNQF && kNQP == kDiagNQP, "diagnostics layout");
 
 int diagnose_partials(pic_ctx *ctx, double *pf, unsigned long long *counts) {
+  {
+    int rc = settle_sorts(ctx);
+    if (rc) return rc;
+  }
   const pic_config &c = ctx->cfg;
   const Geom &g = ctx->geom;
   double *pp[2] = {pf + (int64_t)kDiagNBF * kNQF,
@@ -2078,6 +2196,10 @@ int diagnose(pic_ctx *ctx, pic_diag *out) {
 }
 
 int upload_particles(pic_ctx *ctx, int sp, int64_t n, const double *const *h) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const double dx = ctx->cfg.dx;
   const Geom &g = ctx->geom;
   const int64_t nn = n > 0 ? n : 1;
@@ -2151,6 +2273,10 @@ static int download_raw(pic_ctx *ctx, int sp, int32_t **cell, float **f,
 
 int download_particles(pic_ctx *ctx, int sp, int64_t cap, int64_t *n,
                        double *const *h) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   int32_t *cell;
   float *f;
   int64_t ns;
@@ -2185,6 +2311,10 @@ int download_particles(pic_ctx *ctx, int sp, int64_t cap, int64_t *n,
 
 int download_raw32(pic_ctx *ctx, int sp, int64_t cap, int64_t *n,
                    int32_t *cell_out, float *const *fo) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   int32_t *cell;
   float *f;
   int64_t ns;

@@ -117,6 +117,8 @@ struct pic_ctx {
 
   // scratch for compaction and sorting (sized to the particle capacity)
   uint32_t *keys = nullptr, *keys_alt = nullptr;
+  // idx: sort and selection output; idx_alt: the ions' sort permutation
+  // while it waits for their fused pass (FP32 form, inside pic_step)
   uint32_t *idx = nullptr, *idx_alt = nullptr;
   uint8_t *flags = nullptr;
   void *prim_tmp = nullptr;        // radix sort and selection temporaries
@@ -284,6 +286,13 @@ int push(pic_ctx *, int sp, int do_push_j, int do_drift_n, int do_boundary);
 int particle_boundary(pic_ctx *, int sp);
 int inflow(pic_ctx *, int sp, uint32_t event);
 int sort(pic_ctx *, int sp);
+// pic_step's tile sort: the permutation is left for the species' next fused
+// pass, which reads the particles through it (no copy); settle applies a
+// left permutation with a copy, settle_sorts for both species
+int sort_for_pass(pic_ctx *, int sp);
+int settle(pic_ctx *, int sp);
+int settle_sorts(pic_ctx *);
+bool pending(const pic_ctx *, int sp);
 int diagnose(pic_ctx *, pic_diag *);
 int count_live(pic_ctx *, int sp, int64_t *n);
 int upload_particles(pic_ctx *, int sp, int64_t n, const double *const *h);
