@@ -79,10 +79,11 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None,
     if workload != "2d3v" and thrust:
         # thrust_fave every step (:926), as the reference's step loop
         sim.thrust_ave(True)
-    # live per-kernel timing over every step of the run (HIP events on the
-    # library's stream, a few microseconds per step), so the kernel average
+    # live timing of the particle passes over every step of the run (HIP
+    # events on the library's stream around the passes only: an event record
+    # around every launch would add ~0.1 ms per step), so the pass average
     # covers the same launches a kernel trace of this command does
-    sim.timing(True)
+    sim.timing("passes")
     sim.timing_reset()
     sim.step(warmup)
     sim.sync()
@@ -94,8 +95,14 @@ def run_form(ps, form, steps, warmup, workload="3d", poisson_every=0, tile=None,
     n1 = sim.particle_count(0) + sim.particle_count(1)
     particles = 0.5 * (n0 + n1)
     rate = particles * steps / (t1 - t0)
-    tsteps = warmup + steps
     push_ms, push_n = sim.timing_get("push")
+    # where the step's time goes: every class timed over 20 further steps
+    # (one sort cadence), after the timed region
+    tsteps = 20
+    sim.timing(True)
+    sim.timing_reset()
+    sim.step(tsteps)
+    sim.sync()
     tot = {k: sim.timing_get(k)
            for k in ("push", "push_sort", "field_b", "field_e", "sort", "inflow", "other")}
     sim.timing(False)

@@ -282,6 +282,9 @@ int pic_set_counters(pic_ctx *ctx, int64_t step, int32_t c_add, int32_t c_sort);
 #define PIC_T_PUSH_SORT 6 /* fused particle pass applying pic_step's sort
                              permutation (FP32 form) */
 #define PIC_T_COUNT 7
+/* on: 0 off, 1 every class, 2 the particle-pass classes only (PIC_T_PUSH,
+ * PIC_T_PUSH_SORT); every event record adds a packet to the stream, so mode
+ * 2 leaves the step's other launches back to back. */
 int pic_timing_enable(pic_ctx *ctx, int on);
 /* Total milliseconds and launch count accumulated for a timing class. */
 int pic_timing_get(pic_ctx *ctx, int which, double *ms, int64_t *launches);

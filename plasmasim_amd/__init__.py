@@ -33,6 +33,10 @@ class PicError(RuntimeError):
         self.code = code
 
 
+def _timing_mode(on):
+    return 2 if on == "passes" else (1 if on else 0)
+
+
 def preset(n):
     """Workload preset n (0 tiny, 1 = 32^3, 2 = 128^3, 4 = 64^3, 5 = 256^3)."""
     c = PicConfig()
@@ -248,7 +252,9 @@ class Simulation:
 
     # -- timing -------------------------------------------------------------
     def timing(self, on=True):
-        self._check(self._L.pic_timing_enable(self._ctx, 1 if on else 0))
+        """HIP-event timing per class: True every class, "passes" the
+        particle passes only (fewer packets in the stream), False off."""
+        self._check(self._L.pic_timing_enable(self._ctx, _timing_mode(on)))
 
     def timing_reset(self):
         self._check(self._L.pic_timing_reset(self._ctx))
@@ -388,7 +394,9 @@ class Simulation2D:
         return out
 
     def timing(self, on=True):
-        self._check(self._L.pic2d_timing_enable(self._ctx, 1 if on else 0))
+        """HIP-event timing per class: True every class, "passes" the
+        particle passes only (fewer packets in the stream), False off."""
+        self._check(self._L.pic2d_timing_enable(self._ctx, _timing_mode(on)))
 
     def timing_reset(self):
         self._check(self._L.pic2d_timing_reset(self._ctx))

@@ -456,7 +456,7 @@ extern "C" int pic2d_load_particles(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_renew_grid(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_OTHER);
   int rc = renew_grid(ctx);
   pic::timing_end(&ctx->rt, PIC_T_OTHER);
   return rc;
@@ -466,7 +466,7 @@ extern "C" int pic2d_inflow(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   const uint32_t ev = ++ctx->inflow_event;
   int rc = 0;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_INFLOW);
   for (int sp = 0; sp < 2 && !rc; ++sp) rc = inflow(ctx, sp, ev);
   pic::timing_end(&ctx->rt, PIC_T_INFLOW);
   return rc;
@@ -474,7 +474,7 @@ extern "C" int pic2d_inflow(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_renew_b_half(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_FIELD_B);
   int rc = renew_b_half(ctx);
   pic::timing_end(&ctx->rt, PIC_T_FIELD_B);
   return rc;
@@ -482,7 +482,7 @@ extern "C" int pic2d_renew_b_half(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_halo(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_OTHER);
   int rc = halo(ctx);
   pic::timing_end(&ctx->rt, PIC_T_OTHER);
   return rc;
@@ -490,7 +490,7 @@ extern "C" int pic2d_halo(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_push_deposit_j(pic2d_ctx *ctx, int species) {
   if (!ctx || species < 0 || species > 1) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_PUSH);
   int rc = push(ctx, species, 1, 0, 0, 0);
   pic::timing_end(&ctx->rt, PIC_T_PUSH);
   return rc;
@@ -498,7 +498,7 @@ extern "C" int pic2d_push_deposit_j(pic2d_ctx *ctx, int species) {
 
 extern "C" int pic2d_fold_current(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_OTHER);
   int rc = fold(ctx, PIC_JIX, 6);
   pic::timing_end(&ctx->rt, PIC_T_OTHER);
   return rc;
@@ -511,7 +511,7 @@ extern "C" int pic2d_external_current(pic2d_ctx *ctx, int64_t step) {
 
 extern "C" int pic2d_drift_deposit_n(pic2d_ctx *ctx, int species) {
   if (!ctx || species < 0 || species > 1) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_PUSH);
   int rc = push(ctx, species, 0, 1, 0, 0);
   pic::timing_end(&ctx->rt, PIC_T_PUSH);
   return rc;
@@ -519,7 +519,7 @@ extern "C" int pic2d_drift_deposit_n(pic2d_ctx *ctx, int species) {
 
 extern "C" int pic2d_fold_density(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_OTHER);
   int rc = fold(ctx, PIC_NI, 2);
   pic::timing_end(&ctx->rt, PIC_T_OTHER);
   return rc;
@@ -527,7 +527,7 @@ extern "C" int pic2d_fold_density(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_renew_e(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_FIELD_E);
   int rc = renew_e(ctx);
   pic::timing_end(&ctx->rt, PIC_T_FIELD_E);
   return rc;
@@ -535,7 +535,7 @@ extern "C" int pic2d_renew_e(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_poisson(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_FIELD_E);
   int rc = poisson(ctx);
   pic::timing_end(&ctx->rt, PIC_T_FIELD_E);
   return rc;
@@ -544,7 +544,7 @@ extern "C" int pic2d_poisson(pic2d_ctx *ctx) {
 extern "C" int pic2d_particle_boundary(pic2d_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   int rc = 0;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_OTHER);
   for (int sp = 0; sp < 2 && !rc; ++sp) rc = particle_boundary(ctx, sp);
   pic::timing_end(&ctx->rt, PIC_T_OTHER);
   return rc;
@@ -552,7 +552,7 @@ extern "C" int pic2d_particle_boundary(pic2d_ctx *ctx) {
 
 extern "C" int pic2d_sort(pic2d_ctx *ctx, int species) {
   if (!ctx || species < 0 || species > 1) return PIC_E_ARG;
-  pic::timing_begin(&ctx->rt);
+  pic::timing_begin(&ctx->rt, PIC_T_SORT);
   int rc = sort(ctx, species);
   pic::timing_end(&ctx->rt, PIC_T_SORT);
   return rc;
@@ -609,7 +609,7 @@ extern "C" int pic2d_step(pic2d_ctx *ctx, int nsteps) {
     TRY(pic2d_renew_b_half(ctx));
     TRY(pic2d_halo(ctx));
     for (int sp = 0; sp < 2; ++sp) {
-      pic::timing_begin(&ctx->rt);
+      pic::timing_begin(&ctx->rt, PIC_T_PUSH);
       rc = push(ctx, sp, 1, 1, 1, 0);
       pic::timing_end(&ctx->rt, PIC_T_PUSH);
       if (rc) return rc;

@@ -45,14 +45,22 @@ StepConst step_const(const pic_ctx *ctx, int sp) {
   return k;
 }
 
-void timing_begin(pic_ctx *ctx) {
-  if (!ctx->timing) return;
+// mode 1 times every class, mode 2 the particle passes only (each event
+// record is one more packet in the stream: the passes alone keep the
+// step's other launches back to back)
+static bool timed(const pic_ctx *ctx, int which) {
+  return ctx->timing == 1 ||
+         (ctx->timing == 2 && (which == PIC_T_PUSH || which == PIC_T_PUSH_SORT));
+}
+
+void timing_begin(pic_ctx *ctx, int which) {
+  if (!timed(ctx, which)) return;
   if (ctx->tpending == pic_ctx::kEventPool) timing_flush(ctx);
   (void)hipEventRecord(ctx->tev[0][ctx->tpending], ctx->stream);
 }
 
 void timing_end(pic_ctx *ctx, int which) {
-  if (!ctx->timing) return;
+  if (!timed(ctx, which)) return;
   (void)hipEventRecord(ctx->tev[1][ctx->tpending], ctx->stream);
   ctx->tclass[ctx->tpending] = which;
   ctx->tpending++;
@@ -678,7 +686,7 @@ extern "C" int pic_load_particles(pic_ctx *ctx) {
 
 extern "C" int pic_renew_grid(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_OTHER);
   int rc = DISPATCH(renew_grid, ctx);
   if (!rc && ctx->phi) rc = poisson_zero_phi(ctx);  // renew_grid zeroes phi (:2552)
   timing_end(ctx, PIC_T_OTHER);
@@ -689,7 +697,7 @@ extern "C" int pic_poisson(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   if (!ctx->phi)
     return set_error(ctx, PIC_E_STATE, "poisson: disabled (poisson_every = 0)");
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_FIELD_E);
   int rc = poisson(ctx);
   timing_end(ctx, PIC_T_FIELD_E);
   if (rc) return rc;
@@ -700,7 +708,7 @@ extern "C" int pic_renew_b_half(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   int rc;
   if (!is_f64(ctx) && (rc = f32::finish_deposit(ctx))) return rc;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_FIELD_B);
   rc = DISPATCH(renew_b_half, ctx);
   timing_end(ctx, PIC_T_FIELD_B);
   return rc;
@@ -708,7 +716,7 @@ extern "C" int pic_renew_b_half(pic_ctx *ctx) {
 
 extern "C" int pic_halo_b(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_OTHER);
   int rc = DISPATCH(halo, ctx, PIC_BX);
   timing_end(ctx, PIC_T_OTHER);
   return rc;
@@ -716,7 +724,7 @@ extern "C" int pic_halo_b(pic_ctx *ctx) {
 
 extern "C" int pic_halo_e(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_OTHER);
   int rc = DISPATCH(halo, ctx, PIC_EX);
   timing_end(ctx, PIC_T_OTHER);
   return rc;
@@ -724,7 +732,7 @@ extern "C" int pic_halo_e(pic_ctx *ctx) {
 
 extern "C" int pic_push_deposit_j(pic_ctx *ctx, int species) {
   if (!ctx || species < 0 || species > 1) return PIC_E_ARG;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_PUSH);
   int rc = DISPATCH(push, ctx, species, 1, 0, 0);
   timing_end(ctx, PIC_T_PUSH);
   return rc;
@@ -732,7 +740,7 @@ extern "C" int pic_push_deposit_j(pic_ctx *ctx, int species) {
 
 extern "C" int pic_drift_deposit_n(pic_ctx *ctx, int species) {
   if (!ctx || species < 0 || species > 1) return PIC_E_ARG;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_PUSH);
   int rc = DISPATCH(push, ctx, species, 0, 1, 0);
   timing_end(ctx, PIC_T_PUSH);
   return rc;
@@ -742,7 +750,7 @@ extern "C" int pic_fold_current(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   int rc;
   if (!is_f64(ctx) && (rc = f32::finish_deposit(ctx))) return rc;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_OTHER);
   rc = DISPATCH(fold_current, ctx);
   timing_end(ctx, PIC_T_OTHER);
   return rc;
@@ -752,7 +760,7 @@ extern "C" int pic_renew_e(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   int rc;
   if (!is_f64(ctx) && (rc = f32::finish_deposit(ctx))) return rc;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_FIELD_E);
   rc = DISPATCH(renew_e, ctx);
   timing_end(ctx, PIC_T_FIELD_E);
   return rc;
@@ -761,7 +769,7 @@ extern "C" int pic_renew_e(pic_ctx *ctx) {
 extern "C" int pic_particle_boundary(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   int rc = 0;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_OTHER);
   for (int sp = 0; sp < 2 && !rc; ++sp) rc = DISPATCH(particle_boundary, ctx, sp);
   timing_end(ctx, PIC_T_OTHER);
   return rc;
@@ -771,7 +779,7 @@ extern "C" int pic_inflow(pic_ctx *ctx) {
   if (!ctx) return PIC_E_ARG;
   ctx->inflow_event++;
   int rc = 0;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_INFLOW);
   for (int sp = 0; sp < 2 && !rc; ++sp)
     rc = DISPATCH(inflow, ctx, sp, ctx->inflow_event);
   timing_end(ctx, PIC_T_INFLOW);
@@ -780,7 +788,7 @@ extern "C" int pic_inflow(pic_ctx *ctx) {
 
 extern "C" int pic_sort(pic_ctx *ctx, int species) {
   if (!ctx || species < 0 || species > 1) return PIC_E_ARG;
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_SORT);
   int rc = DISPATCH(sort, ctx, species);
   timing_end(ctx, PIC_T_SORT);
   return rc;
@@ -974,7 +982,7 @@ extern "C" int pic_thrust_ave_get(pic_ctx *ctx, double F[27], int64_t *nsteps,
 
 namespace pic {
 int push_fused(pic_ctx *ctx, int sp) {
-  timing_begin(ctx);
+  timing_begin(ctx, PIC_T_PUSH);
   int rc = DISPATCH(push, ctx, sp, 1, 1, 1);
   timing_end(ctx, PIC_T_PUSH);
   return rc;
@@ -1006,7 +1014,7 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
   const auto sort_both = [&]() -> int {
     ctx->c_sort = 0;
     for (int sp = 0; sp < 2; ++sp) {
-      timing_begin(ctx);
+      timing_begin(ctx, PIC_T_SORT);
       const int r = f64 ? f64::sort(ctx, sp) : f32::sort_for_pass(ctx, sp);
       timing_end(ctx, PIC_T_SORT);
       if (r) return r;
@@ -1035,7 +1043,7 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
     TRY(pic_halo_b(ctx));
     for (int sp = 0; sp < 2; ++sp) {
       const int tclass = !f64 && f32::pending(ctx, sp) ? PIC_T_PUSH_SORT : PIC_T_PUSH;
-      timing_begin(ctx);
+      timing_begin(ctx, tclass);
       rc = DISPATCH(push, ctx, sp, 1, 1, 1);
       timing_end(ctx, tclass);
       if (rc) {
@@ -1069,12 +1077,11 @@ extern "C" int pic_step(pic_ctx *ctx, int nsteps) {
 // timing
 
 extern "C" int pic_timing_enable(pic_ctx *ctx, int on) {
-  if (!ctx) return PIC_E_ARG;
-  if (!on) {
-    int rc = timing_flush(ctx);
-    if (rc) return rc;
-  }
-  ctx->timing = on ? 1 : 0;
+  if (!ctx || on < 0 || on > 2) return PIC_E_ARG;
+  // events recorded under the old mode are collected first
+  int rc = timing_flush(ctx);
+  if (rc) return rc;
+  ctx->timing = on;
   return 0;
 }
 

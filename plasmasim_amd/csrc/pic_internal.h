@@ -178,7 +178,7 @@ struct pic_ctx {
 
   // timing: pairs of HIP events around launches, resolved in batches
   static constexpr int kEventPool = 512;
-  int timing = 0;
+  int timing = 0;  // 0 off, 1 every class, 2 the particle passes only
   double t_ms[PIC_T_COUNT] = {0};
   int64_t t_n[PIC_T_COUNT] = {0};
   hipEvent_t tev[2][kEventPool] = {};
@@ -254,7 +254,7 @@ inline TileGeom tile_geom(const pic_ctx *ctx) {
 }
 
 // timing helpers (HIP events on the context stream)
-void timing_begin(pic_ctx *ctx);
+void timing_begin(pic_ctx *ctx, int which);
 void timing_end(pic_ctx *ctx, int which);
 int timing_flush(pic_ctx *ctx);
 
