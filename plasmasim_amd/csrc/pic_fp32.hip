@@ -1953,14 +1953,15 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
     int *over = (int *)ctx->keys;  // free between sorts
     k_slab_count32<<<(unsigned)nslab, kBlock, 0, ctx->stream>>>(r, P.cell, over);
     k_slab_scan32<<<1, kBlock, 0, ctx->stream>>>(over, nslab);
+    const size_t lds = (size_t)tg.TX * tg.TY * 2 * c.ppc * sizeof(int);
+    k_slab_refill32<<<(unsigned)nslab, kBlock, lds, ctx->stream>>>(r, P, over, ctx->n[sp]);
     PIC_LAUNCH_CHECK(ctx);
+    // the refill takes its offsets from the scan on the device; the host
+    // reads the appended count after launching it, so the GPU stays busy
     PIC_HIP(ctx, hipMemcpyAsync(ctx->host_counter, over + nslab, sizeof(int),
                                 hipMemcpyDeviceToHost, ctx->stream));
     PIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const int64_t appended = *(int *)ctx->host_counter;
-    const size_t lds = (size_t)tg.TX * tg.TY * 2 * c.ppc * sizeof(int);
-    k_slab_refill32<<<(unsigned)nslab, kBlock, lds, ctx->stream>>>(r, P, over, ctx->n[sp]);
-    PIC_LAUNCH_CHECK(ctx);
     ctx->n[sp] += appended;
     return 0;
   }
