@@ -1981,14 +1981,14 @@ int inflow(pic_ctx *ctx, int sp, uint32_t event) {
 int settle(pic_ctx *ctx, int sp) {
   State32 *S = st(ctx);
   if (!S || !S->pending[sp]) return 0;
-  S->pending[sp] = false;
   const int64_t n = ctx->n[sp];
   Particles32 src = p32(ctx, sp, ctx->cur[sp]);
   Particles32 dst = p32(ctx, sp, 1 - ctx->cur[sp]);
   k_gather32<<<(unsigned)div_up(n, 4 * kBlock), kBlock, 0, ctx->stream>>>(
       dst, src, pass_perm(ctx, sp), n);
-  PIC_LAUNCH_CHECK(ctx);
+  PIC_LAUNCH_CHECK(ctx);  // a failed launch leaves the permutation waiting
   ctx->cur[sp] = 1 - ctx->cur[sp];
+  S->pending[sp] = false;
   return 0;
 }
 
