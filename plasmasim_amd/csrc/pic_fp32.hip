@@ -239,20 +239,20 @@ __device__ inline void glob_flush(u64 *p, u64 v, double) {
 
 // the lane's walk over the 8 nodes of a stencil in the LDS window
 struct Walk {
-  bool mx, my, mz;  // axes walked downwards
   int first;        // byte offset of the first visited node from the stencil base
   int off[8];       // byte offsets of the visits from the first node
+  // an axis's two weights in walk order are s x + b and -s x + c for the
+  // weight x of its upper node: (1 - x, x), or (x, 1 - x) walked downwards
+  float s[3], b[3], c[3];
 };
 
 __device__ inline Walk make_walk(int WY, int WZ) {
   Walk k;
   const int sw = threadIdx.x & 7;
-  k.mx = sw & 4;
-  k.my = sw & 2;
-  k.mz = sw & 1;
+  const bool mx = sw & 4, my = sw & 2, mz = sw & 1;  // axes walked downwards
   const int sz = 8, sy = 8 * WZ, sx = 8 * WY * WZ;
-  const int dx = k.mx ? -sx : sx, dy = k.my ? -sy : sy, dz = k.mz ? -sz : sz;
-  k.first = (k.mx ? sx : 0) + (k.my ? sy : 0) + (k.mz ? sz : 0);
+  const int dx = mx ? -sx : sx, dy = my ? -sy : sy, dz = mz ? -sz : sz;
+  k.first = (mx ? sx : 0) + (my ? sy : 0) + (mz ? sz : 0);
   k.off[0] = 0;
   k.off[1] = dz;
   k.off[2] = dy;
@@ -261,7 +261,21 @@ __device__ inline Walk make_walk(int WY, int WZ) {
   k.off[5] = dx + dz;
   k.off[6] = dx + dy;
   k.off[7] = dx + dy + dz;
+  const bool mm[3] = {mx, my, mz};
+  for (int a = 0; a < 3; ++a) {
+    k.s[a] = mm[a] ? 1.0f : -1.0f;
+    k.b[a] = mm[a] ? 0.0f : 1.0f;
+    k.c[a] = mm[a] ? 1.0f : 0.0f;
+  }
   return k;
+}
+
+// the weights (1 - x, x) of an axis in the lane's walk order; each is one
+// rounding of the same value as the plain 1 - x and x
+__device__ __forceinline__ void walk_pair(const Walk &k, int a, float x, float &w0,
+                                          float &w1) {
+  w0 = __builtin_fmaf(k.s[a], x, k.b[a]);
+  w1 = __builtin_fmaf(-k.s[a], x, k.c[a]);
 }
 
 // a contribution known to be non-negative (the density: weight and shape
@@ -274,31 +288,6 @@ __device__ __forceinline__ void lds_fx_add_pos(char *acc, int byte, float v) {
 __device__ __forceinline__ void lds_fx_add(char *acc, int byte, float v) {
   __hip_atomic_fetch_add((PIC_LDS u64 *)(acc + byte), to_fx(v), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Deposit q * a_i b_j c_k at the 8 window nodes of a stencil whose lowest
-// node is at byte offset base; q carries the fixed-point scale.
-template <bool POS>
-__device__ __forceinline__ void lds_dep8(char *acc, int base, const Walk &k,
-                                         float q, float a0, float a1, float b0,
-                                         float b1, float c0, float c1) {
-  const float A0 = k.mx ? a1 : a0, A1 = k.mx ? a0 : a1;
-  const float B0 = k.my ? b1 : b0, B1 = k.my ? b0 : b1;
-  const float C0 = k.mz ? c1 : c0, C1 = k.mz ? c0 : c1;
-  const float qa0 = q * A0, qa1 = q * A1;
-  const float bc00 = B0 * C0, bc01 = B0 * C1, bc10 = B1 * C0, bc11 = B1 * C1;
-  const int p = base + k.first;
-  const auto add = [&](int byte, float v) {
-    if (POS) lds_fx_add_pos(acc, byte, v); else lds_fx_add(acc, byte, v);
-  };
-  add(p + k.off[0], qa0 * bc00);
-  add(p + k.off[1], qa0 * bc01);
-  add(p + k.off[2], qa0 * bc10);
-  add(p + k.off[3], qa0 * bc11);
-  add(p + k.off[4], qa1 * bc00);
-  add(p + k.off[5], qa1 * bc01);
-  add(p + k.off[6], qa1 * bc10);
-  add(p + k.off[7], qa1 * bc11);
 }
 
 // the same stencil straight into the global planes (particles outside the
@@ -369,6 +358,29 @@ __device__ inline bool in_win_s(const WinS &w, int k, int l, int m) {
 
 // J (current) or n deposit of one particle at cell (k,l,m), offsets
 // (ox,oy,oz), into the workgroup's LDS window
+// Deposit q * a_i b_j c_k at the 8 window nodes of a stencil whose lowest
+// node is at byte offset base, visiting them in the lane's walk order; the
+// weights come in that order (walk_pair) and q carries the fixed-point scale.
+template <bool POS>
+__device__ __forceinline__ void lds_dep8(char *acc, int base, const Walk &k, float q,
+                                         float A0, float A1, float B0, float B1,
+                                         float C0, float C1) {
+  const float qa0 = q * A0, qa1 = q * A1;
+  const float bc00 = B0 * C0, bc01 = B0 * C1, bc10 = B1 * C0, bc11 = B1 * C1;
+  const int p = base + k.first;
+  const auto add = [&](int byte, float v) {
+    if (POS) lds_fx_add_pos(acc, byte, v); else lds_fx_add(acc, byte, v);
+  };
+  add(p + k.off[0], qa0 * bc00);
+  add(p + k.off[1], qa0 * bc01);
+  add(p + k.off[2], qa0 * bc10);
+  add(p + k.off[3], qa0 * bc11);
+  add(p + k.off[4], qa1 * bc00);
+  add(p + k.off[5], qa1 * bc01);
+  add(p + k.off[6], qa1 * bc10);
+  add(p + k.off[7], qa1 * bc11);
+}
+
 __device__ __forceinline__ void deposit_lds(const WinS &w, const Walk &wk, char *acc,
                                             bool current, int k, int l, int m,
                                             float ox, float oy, float oz, float vx,
@@ -377,17 +389,25 @@ __device__ __forceinline__ void deposit_lds(const WinS &w, const Walk &wk, char 
   const int sl = 8 * w.WZ, sk = 8 * w.WY * w.WZ;
   const int c0 = (((k - w.k0) * w.WY + (l - w.l0)) * w.WZ + (m - w.m0)) * 8;
   const int hx = X.bh != k ? sk : 0, hy = Y.bh != l ? sl : 0, hz = Z.bh != m ? 8 : 0;
+  float xh0, xh1, yh0, yh1, zh0, zh1;
+  walk_pair(wk, 0, X.h1, xh0, xh1);
+  walk_pair(wk, 1, Y.h1, yh0, yh1);
+  walk_pair(wk, 2, Z.h1, zh0, zh1);
   if (current) {
     const float qs = kp.qw * kp.fx;
-    lds_dep8<false>(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, X.i0, X.i1, Y.h0,
-                    Y.h1, Z.h0, Z.h1);
-    lds_dep8<false>(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, X.h0, X.h1, Y.i0,
-                    Y.i1, Z.h0, Z.h1);
-    lds_dep8<false>(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, X.h0, X.h1, Y.h0,
-                    Y.h1, Z.i0, Z.i1);
+    float xi0, xi1, yi0, yi1, zi0, zi1;
+    walk_pair(wk, 0, X.i1, xi0, xi1);
+    walk_pair(wk, 1, Y.i1, yi0, yi1);
+    walk_pair(wk, 2, Z.i1, zi0, zi1);
+    lds_dep8<false>(acc + 0 * 8 * kWin, c0 - hy - hz, wk, qs * vx, xi0, xi1, yh0, yh1,
+                    zh0, zh1);
+    lds_dep8<false>(acc + 1 * 8 * kWin, c0 - hx - hz, wk, qs * vy, xh0, xh1, yi0, yi1,
+                    zh0, zh1);
+    lds_dep8<false>(acc + 2 * 8 * kWin, c0 - hx - hy, wk, qs * vz, xh0, xh1, yh0, yh1,
+                    zi0, zi1);
   } else {
-    lds_dep8<true>(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, X.h0, X.h1,
-                   Y.h0, Y.h1, Z.h0, Z.h1);
+    lds_dep8<true>(acc + 3 * 8 * kWin, c0 - hx - hy - hz, wk, kp.w * kp.fx, xh0, xh1,
+                   yh0, yh1, zh0, zh1);
   }
 }
 
