@@ -591,9 +591,13 @@ extern "C" int pic2d_step(pic2d_ctx *ctx, int nsteps) {
   if (!ctx || nsteps < 0) return PIC_E_ARG;
   const pic2d_config &c = ctx->cfg;
   int rc;
-#define TRY(x)                 \
-  do {                         \
-    if ((rc = (x))) return rc; \
+  // an error leaves no sort permutation waiting for a pass
+#define TRY(x)                              \
+  do {                                      \
+    if ((rc = (x))) {                       \
+      (void)settle_sorts(ctx);              \
+      return rc;                            \
+    }                                       \
   } while (0)
   for (int s = 0; s < nsteps; ++s) {
     TRY(pic2d_renew_grid(ctx));
@@ -602,17 +606,25 @@ extern "C" int pic2d_step(pic2d_ctx *ctx, int nsteps) {
       ctx->c_add = 0;
     }
     if (c.sort_every > 0 && ctx->c_sort >= c.sort_every) {
-      TRY(pic2d_sort(ctx, PIC_ION));
-      TRY(pic2d_sort(ctx, PIC_ELECTRON));
+      // the FP32 form's tile sort leaves its permutation to the species'
+      // fused pass below, which reads the particles through it (no copy);
+      // only field work lies between them
+      for (int sp = 0; sp < 2; ++sp) {
+        pic::timing_begin(&ctx->rt, PIC_T_SORT);
+        rc = ctx->f64 ? sort(ctx, sp) : sort_for_pass(ctx, sp);
+        pic::timing_end(&ctx->rt, PIC_T_SORT);
+        TRY(rc);
+      }
       ctx->c_sort = 0;
     }
     TRY(pic2d_renew_b_half(ctx));
     TRY(pic2d_halo(ctx));
     for (int sp = 0; sp < 2; ++sp) {
-      pic::timing_begin(&ctx->rt, PIC_T_PUSH);
+      const int tclass = ctx->pending[sp] ? PIC_T_PUSH_SORT : PIC_T_PUSH;
+      pic::timing_begin(&ctx->rt, tclass);
       rc = push(ctx, sp, 1, 1, 1, 0);
-      pic::timing_end(&ctx->rt, PIC_T_PUSH);
-      if (rc) return rc;
+      pic::timing_end(&ctx->rt, tclass);
+      TRY(rc);
     }
     TRY(pic2d_fold_current(ctx));
     TRY(pic2d_external_current(ctx, ctx->step));

@@ -97,6 +97,11 @@ struct pic2d_ctx {
   // sorts took the tile sort and the radix sort
   pic::tsort::Bufs *ts = nullptr;
   int64_t sort_tile_count = 0, sort_radix_count = 0;
+  // pic2d_step's tile sort leaves its permutation to the species' fused
+  // pass of the same step (pending), which reads the particles through it:
+  // the ions' waits in perm_ion, the electrons' in rt.idx
+  uint32_t *perm_ion = nullptr;
+  bool pending[2] = {false, false};
 };
 
 namespace pic2d {
@@ -143,6 +148,12 @@ int push(pic2d_ctx *, int sp, int push_j, int drift_n, int boundary,
 int particle_boundary(pic2d_ctx *, int sp);
 int inflow(pic2d_ctx *, int sp, uint32_t event);
 int sort(pic2d_ctx *, int sp);
+// the FP32 tile sort as pic2d_step runs it: the permutation is left to the
+// species' next fused pass (no copy); settle applies a left permutation
+// with a copy, settle_sorts for both species
+int sort_for_pass(pic2d_ctx *, int sp);
+int settle(pic2d_ctx *, int sp);
+int settle_sorts(pic2d_ctx *);
 int tsort_setup(pic2d_ctx *);  // the tile sort's lists, at create
 void tsort_free(pic2d_ctx *);
 int count_live(pic2d_ctx *, int sp, int64_t *n);

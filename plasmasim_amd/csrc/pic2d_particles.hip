@@ -690,10 +690,13 @@ __device__ __forceinline__ void step32(const Geom2 &g, const F32Fields &F,
   cl = dead ? -1 : k * g.SZ + m;
 }
 
-template <bool PUSH_J, bool DRIFT, bool DEP_N, bool BOUNDARY>
+// PERM: the pass also applies a sort's permutation: slot i takes its
+// particle from slot perm[i] of Q and stores the result in P (the other
+// buffer of the species), so the sort needs no separate copy
+template <bool PUSH_J, bool DRIFT, bool DEP_N, bool BOUNDARY, bool PERM>
 __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
-                                                P32 P, Run2 tr, K32 kp,
-                                                Dep32 D) {
+                                                P32 P, Run2 tr, K32 kp, Dep32 D,
+                                                P32 Q, const uint32_t *__restrict__ perm) {
   __shared__ u64 acc[4 * kWin2];
   __shared__ Win2 sw;
   __shared__ int64_t s_range[2];
@@ -704,7 +707,7 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
     Win2 w;
     setup2(bid, tr, tg,
            [&](int64_t i, int &k, int &m) {
-             const int32_t c = P.cell[i];
+             const int32_t c = PERM ? Q.cell[perm[i]] : P.cell[i];
              if (c < 0) return false;
              split_cell(g, kp, c, k, m);
              return true;
@@ -730,16 +733,38 @@ __global__ __launch_bounds__(256) void k_pass32(Geom2 g, Tile2 tg, F32Fields F,
   const int64_t q0 = s >> 2, q1 = (e + 3) >> 2;
   for (int64_t q = q0 + tid; q < q1; q += blockDim.x) {
     const int64_t i0 = q << 2;
-    const int4 c4 = *(const int4 *)(P.cell + i0);
-    const float4 x4 = *(const float4 *)(P.ox + i0);
-    const float4 z4 = *(const float4 *)(P.oz + i0);
-    const float4 u4 = *(const float4 *)(P.vx + i0);
-    const float4 v4 = *(const float4 *)(P.vy + i0);
-    const float4 w4 = *(const float4 *)(P.vz + i0);
-    int cs[4] = {c4.x, c4.y, c4.z, c4.w};
-    float xs[4] = {x4.x, x4.y, x4.z, x4.w}, zs[4] = {z4.x, z4.y, z4.z, z4.w},
-          us[4] = {u4.x, u4.y, u4.z, u4.w}, vs[4] = {v4.x, v4.y, v4.z, v4.w},
-          ws[4] = {w4.x, w4.y, w4.z, w4.w};
+    int cs[4];
+    float xs[4], zs[4], us[4], vs[4], ws[4];
+    if (PERM) {
+      // the capacity is a multiple of 256, so the quad's perm entries are
+      // allocated; slots outside the range read slot 0 and are not stored
+      const uint4 p4 = *(const uint4 *)(perm + i0);
+      const uint32_t pj[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t i = i0 + j;
+        const uint32_t o = i >= s && i < e ? pj[j] : 0u;
+        cs[j] = Q.cell[o];
+        xs[j] = Q.ox[o];
+        zs[j] = Q.oz[o];
+        us[j] = Q.vx[o];
+        vs[j] = Q.vy[o];
+        ws[j] = Q.vz[o];
+      }
+    } else {
+      const int4 c4 = *(const int4 *)(P.cell + i0);
+      const float4 x4 = *(const float4 *)(P.ox + i0);
+      const float4 z4 = *(const float4 *)(P.oz + i0);
+      const float4 u4 = *(const float4 *)(P.vx + i0);
+      const float4 v4 = *(const float4 *)(P.vy + i0);
+      const float4 w4 = *(const float4 *)(P.vz + i0);
+      cs[0] = c4.x, cs[1] = c4.y, cs[2] = c4.z, cs[3] = c4.w;
+      xs[0] = x4.x, xs[1] = x4.y, xs[2] = x4.z, xs[3] = x4.w;
+      zs[0] = z4.x, zs[1] = z4.y, zs[2] = z4.z, zs[3] = z4.w;
+      us[0] = u4.x, us[1] = u4.y, us[2] = u4.z, us[3] = u4.w;
+      vs[0] = v4.x, vs[1] = v4.y, vs[2] = v4.z, vs[3] = v4.w;
+      ws[0] = w4.x, ws[1] = w4.y, ws[2] = w4.z, ws[3] = w4.w;
+    }
     const bool full = i0 >= s && i0 + 3 < e;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -826,8 +851,16 @@ static Run2 run_of(pic2d_ctx *ctx, int sp, unsigned *nblocks) {
   return tr;
 }
 
+static uint32_t *pass_perm(pic2d_ctx *ctx, int sp);
+
 int push(pic2d_ctx *ctx, int sp, int push_j, int drift_n, int boundary,
          int density_only) {
+  // a pending sort permutation is applied by the fused pass only
+  const bool fused = push_j && drift_n && boundary && !density_only;
+  if (!fused) {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   if (ctx->n[sp] == 0) return 0;
   unsigned nb;
   const Run2 tr = run_of(ctx, sp, &nb);
@@ -859,7 +892,11 @@ int push(pic2d_ctx *ctx, int sp, int push_j, int drift_n, int boundary,
                    plane<float>(ctx, PIC_BY), plane<float>(ctx, PIC_BZ)};
     Dep32 D = {plane<float>(ctx, jx), plane<float>(ctx, jx + 1),
                plane<float>(ctx, jx + 2), plane<float>(ctx, nn)};
-    const P32 P = p32(ctx, sp, ctx->cur[sp]);
+    // with a pending permutation the pass reads the current buffer through
+    // it and writes the other one
+    const uint32_t *pm = ctx->pending[sp] ? pass_perm(ctx, sp) : nullptr;
+    const P32 Q = p32(ctx, sp, ctx->cur[sp]);
+    const P32 P = pm ? p32(ctx, sp, 1 - ctx->cur[sp]) : Q;
     const pic2d_config &c = ctx->cfg;
     const Push2 p = push_const(ctx, sp);
     K32 kp;
@@ -877,9 +914,15 @@ int push(pic2d_ctx *ctx, int sp, int push_j, int drift_n, int boundary,
     kp.fx_inv = 1.0 / ctx->fx_scale;
     cell_magic(ctx->g.SZ, kp.cmag, kp.csh);
 #define L32(a, b, c, d) \
-  k_pass32<a, b, c, d><<<nb, 256, 0, st>>>(ctx->g, ctx->tg, F, P, tr, kp, D)
+  k_pass32<a, b, c, d, false><<<nb, 256, 0, st>>>(ctx->g, ctx->tg, F, P, tr, kp, D, Q, pm)
     switch (sel) {
-      case 7: L32(true, true, true, true); break;
+      case 7:
+        if (pm)
+          k_pass32<true, true, true, true, true>
+              <<<nb, 256, 0, st>>>(ctx->g, ctx->tg, F, P, tr, kp, D, Q, pm);
+        else
+          L32(true, true, true, true);
+        break;
       case 4: L32(true, false, false, false); break;
       case 2: L32(false, true, true, false); break;
       case 1: L32(false, false, false, true); break;
@@ -887,6 +930,12 @@ int push(pic2d_ctx *ctx, int sp, int push_j, int drift_n, int boundary,
       default: return err(ctx, PIC_E_ARG, "push: unsupported routine combination");
     }
 #undef L32
+    P2_LAUNCH_CHECK(ctx);
+    if (pm) {
+      ctx->cur[sp] = 1 - ctx->cur[sp];
+      ctx->pending[sp] = false;
+    }
+    return 0;
   }
   P2_LAUNCH_CHECK(ctx);
   return 0;
@@ -962,6 +1011,10 @@ static int clear_tiles(pic2d_ctx *ctx, int sp) {
 }
 
 int load_particles(pic2d_ctx *ctx) {
+  {
+    int rc = settle_sorts(ctx);
+    if (rc) return rc;
+  }
   const pic2d_config &c = ctx->cfg;
   const int64_t n = (int64_t)c.nx * c.nz * c.ppc;
   if (n > ctx->cap)
@@ -1096,17 +1149,39 @@ __global__ void k_gather2_32(P32 d, P32 s, const uint32_t *__restrict__ idx, int
 }
 
 // dst[i] = src[idx[i]] for i < n in the context's form
-static int gather_perm(pic2d_ctx *ctx, int sp, int src, int dst, int64_t n) {
+static int gather_perm(pic2d_ctx *ctx, int sp, int src, int dst, int64_t n,
+                       const uint32_t *idx) {
   if (n <= 0) return 0;
   if (ctx->f64)
     k_gather2<<<(unsigned)div_up(n, kBlock), kBlock, 0, ctx->rt.stream>>>(
         1, p64(ctx, sp, dst), p32(ctx, sp, dst), p64(ctx, sp, src), p32(ctx, sp, src),
-        ctx->rt.idx, n);
+        idx, n);
   else
     k_gather2_32<<<(unsigned)div_up(n, 4 * kBlock), kBlock, 0, ctx->rt.stream>>>(
-        p32(ctx, sp, dst), p32(ctx, sp, src), ctx->rt.idx, n);
+        p32(ctx, sp, dst), p32(ctx, sp, src), idx, n);
   P2_LAUNCH_CHECK(ctx);
   return 0;
+}
+
+// where a deferred permutation waits for its pass
+static uint32_t *pass_perm(pic2d_ctx *ctx, int sp) {
+  return sp == 0 ? ctx->perm_ion : ctx->rt.idx;
+}
+
+int settle(pic2d_ctx *ctx, int sp) {
+  if (!ctx->pending[sp]) return 0;
+  const int src = ctx->cur[sp];
+  int rc = gather_perm(ctx, sp, src, 1 - src, ctx->n[sp], pass_perm(ctx, sp));
+  if (rc) return rc;  // a failed launch leaves the permutation waiting
+  ctx->cur[sp] = 1 - src;
+  ctx->pending[sp] = false;
+  return 0;
+}
+
+int settle_sorts(pic2d_ctx *ctx) {
+  int rc = settle(ctx, 0);
+  const int re = settle(ctx, 1);
+  return rc ? rc : re;
 }
 
 // stable compaction of species sp into its other buffer; the slots lose
@@ -1123,7 +1198,7 @@ static int compact(pic2d_ctx *ctx, int sp) {
   int64_t live = 0;
   int rc = pic::select_flagged(&ctx->rt, ctx->rt.flags, ctx->rt.idx, n, &live);
   if (rc) return rc;
-  rc = gather_perm(ctx, sp, src, dst, live);
+  rc = gather_perm(ctx, sp, src, dst, live, ctx->rt.idx);
   if (rc) return rc;
   ctx->cur[sp] = dst;
   ctx->n[sp] = live;
@@ -1131,6 +1206,10 @@ static int compact(pic2d_ctx *ctx, int sp) {
 }
 
 int inflow(pic2d_ctx *ctx, int sp, uint32_t event) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const pic2d_config &c = ctx->cfg;
   const int f64 = ctx->f64 ? 1 : 0;
   if (ctx->n[sp] > 0) {
@@ -1221,17 +1300,26 @@ int tsort_setup(pic2d_ctx *ctx) {
     return 0;
   ctx->ts = new (std::nothrow) pic::tsort::Bufs();
   if (!ctx->ts) return err(ctx, PIC_E_NOMEM, "tile sort lists");
+  P2_HIP(ctx, hipMalloc(&ctx->perm_ion, (size_t)ctx->cap * sizeof(uint32_t)));
   return pic::tsort::reserve(&ctx->rt, *ctx->ts, nt, ctx->cap);
 }
 
 void tsort_free(pic2d_ctx *ctx) {
+  if (ctx->perm_ion) (void)hipFree(ctx->perm_ion);
+  ctx->perm_ion = nullptr;
   if (!ctx->ts) return;
   pic::tsort::release(*ctx->ts);
   delete ctx->ts;
   ctx->ts = nullptr;
 }
 
-int sort(pic2d_ctx *ctx, int sp) {
+// defer: a tile sort leaves its permutation for the species' next fused
+// pass instead of copying the particles (pic2d_step, FP32 form)
+static int sort_impl(pic2d_ctx *ctx, int sp, bool defer) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const int64_t n = ctx->n[sp];
   const int nt = ctx->tg.ntiles();
   const int f64 = ctx->f64 ? 1 : 0;
@@ -1243,11 +1331,19 @@ int sort(pic2d_ctx *ctx, int sp) {
     const KeyMap2 km{ctx->g, ctx->tg};
     bool done = false;
     int64_t valid = 0;
+    uint32_t *perm = defer ? pass_perm(ctx, sp) : ctx->rt.idx;
     int rc = pic::tsort::run(&ctx->rt, km, nt, ctx->tile_start[sp], ctx->sorted_end[sp], n,
-                             p32(ctx, sp, src).cell, *ctx->ts, ctx->rt.idx, &done, &valid);
+                             p32(ctx, sp, src).cell, *ctx->ts, perm, &done, &valid);
     if (rc) return rc;
+    if (done && defer && valid > 0) {
+      ctx->n[sp] = valid;
+      ctx->sorted_end[sp] = valid;
+      ctx->sort_tile_count++;
+      ctx->pending[sp] = true;
+      return 0;
+    }
     if (done) {
-      rc = gather_perm(ctx, sp, src, dst, valid);
+      rc = gather_perm(ctx, sp, src, dst, valid, perm);
       if (rc) return rc;
       ctx->cur[sp] = dst;
       ctx->n[sp] = valid;
@@ -1274,7 +1370,7 @@ int sort(pic2d_ctx *ctx, int sp) {
                              sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   P2_HIP(ctx, hipStreamSynchronize(st));
   const int64_t valid = (int64_t)(*(uint32_t *)ctx->rt.host_counter);
-  rc = gather_perm(ctx, sp, src, dst, valid);
+  rc = gather_perm(ctx, sp, src, dst, valid, ctx->rt.idx);
   if (rc) return rc;
   ctx->cur[sp] = dst;
   ctx->n[sp] = valid;
@@ -1302,7 +1398,15 @@ __global__ __launch_bounds__(256) void k_count2(int f64, P64 P, P32 Q, int64_t n
   }
 }
 
+int sort(pic2d_ctx *ctx, int sp) { return sort_impl(ctx, sp, false); }
+
+int sort_for_pass(pic2d_ctx *ctx, int sp) { return sort_impl(ctx, sp, true); }
+
 int count_live(pic2d_ctx *ctx, int sp, int64_t *out) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const int64_t n = ctx->n[sp];
   *out = 0;
   if (n == 0) return 0;
@@ -1352,6 +1456,10 @@ __global__ void k_to64(Geom2 g, P32 Q, double *x, double *z, double *vx,
 }
 
 int upload_particles(pic2d_ctx *ctx, int sp, int64_t n, const double *const *h) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   if (n > ctx->cap)
     return pic::set_error(&ctx->rt, PIC_E_CAPACITY,
                           "upload: %lld particles > capacity %lld", (long long)n,
@@ -1387,6 +1495,10 @@ int upload_particles(pic2d_ctx *ctx, int sp, int64_t n, const double *const *h) 
 
 int download_particles(pic2d_ctx *ctx, int sp, int64_t cap, int64_t *nout,
                        double *const *h) {
+  {
+    int rc = settle(ctx, sp);
+    if (rc) return rc;
+  }
   const int64_t n = ctx->n[sp];
   const int f64 = ctx->f64 ? 1 : 0;
   const int cur = ctx->cur[sp];
@@ -1469,6 +1581,10 @@ __global__ __launch_bounds__(256) void k_pdiag2(int f64, P64 P, P32 Q, int64_t n
 }
 
 int diagnose_particles(pic2d_ctx *ctx, pic2d_diag *d) {
+  {
+    int rc = settle_sorts(ctx);
+    if (rc) return rc;
+  }
   const pic2d_config &c = ctx->cfg;
   const int nbk = 512;
   double *part = nullptr;

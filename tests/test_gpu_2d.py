@@ -261,6 +261,43 @@ def test_fp32_one_step_error_vs_fp64():
         assert a.particle_count(sp) == b.particle_count(sp)
 
 
+def test_fp32_step_sort_applied_by_the_pass_matches_copying_sort():
+    """pic2d_step's FP32 tile sort leaves its permutation to the species'
+    fused pass, which reads the particles through it.  A run sorting inside
+    each step that way and a run sorting with the C-ABI sort (which copies
+    the particles) before the same steps hold the same particles in the same
+    slots after every step: slot-by-slot relative L2 <= 1e-6 (the passes'
+    float atomics add in no fixed order), counts exact, and the fields to
+    the same bar; a wrong permutation moves particles between slots, an
+    error of order one."""
+    runs = []
+    for in_step in (True, False):
+        c = cfg2(1, ps.PIC_FP32, inflow_every=1000, sort_every=1 if in_step else 1000)
+        s = ps.Simulation2D(c)
+        s.load_particles()
+        s.begin()
+        states = []
+        for k in range(5):
+            # pic2d_step sorts when c_sort >= sort_every: from the second step
+            if not in_step and k >= 1:
+                s.sort(0)
+                s.sort(1)
+            s.step(1)
+            states.append([s.download_particles(sp) for sp in (0, 1)])
+        runs.append((states, {n: s.download_field(n) for n in EB + ["ni", "ne"]},
+                     s.sort_stats()))
+        s.close()
+    (pa, fa, sa), (pb, fb, sb) = runs
+    assert sa == sb == (8, 0), (sa, sb)
+    for k in range(5):
+        for sp in (0, 1):
+            a, b = pa[k][sp], pb[k][sp]
+            assert a.shape == b.shape and a.shape[0] > 0, (k, sp)
+            assert rel_l2(a, b) <= 1e-6, (k, sp, rel_l2(a, b))
+    for n in fa:
+        assert rel_l2(fa[n], fb[n]) <= 1e-6, (n, rel_l2(fa[n], fb[n]))
+
+
 def test_fp32_multistep_with_sort_inflow_poisson():
     c = cfg2(1, ps.PIC_FP32, inflow_every=7, sort_every=5)
     s = ps.Simulation2D(c)
