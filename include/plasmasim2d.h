@@ -147,7 +147,9 @@ int pic2d_sort(pic2d_ctx *ctx, int species);
 int pic2d_sort_stats(const pic2d_ctx *ctx, int64_t *tile, int64_t *radix);
 
 /* n whole steps of fdtd's loop with the particle loops and the particle
- * boundary fused into one pass per species */
+ * boundary fused into one pass per species.  FP32 form: a step's tile sort
+ * copies no particles, the species' pass of that step reads them through
+ * the sort's permutation (timed as PIC_T_PUSH_SORT). */
 int pic2d_step(pic2d_ctx *ctx, int nsteps);
 int pic2d_step_index(const pic2d_ctx *ctx, int64_t *step);
 
