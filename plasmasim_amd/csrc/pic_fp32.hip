@@ -17,6 +17,11 @@
 // directly.  Atomic mode: global fp32 atomics into the fp32 planes.
 // Deterministic mode: global 64-bit integer atomics into fixed-point planes,
 // converted to fp32 after the pass, so results are bit-reproducible.
+//
+// Sorting: the tile sort of pic_tsort.h.  Called on its own it copies the
+// particles into their new slots; inside pic_step the species' pass of the
+// sorting step reads them through the permutation instead and writes the
+// other buffer, and nothing else touches the particles in between.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdlib.h>
