@@ -9,7 +9,10 @@
 // deposits whose four nodes fall outside the window go to global memory
 // directly.  The FP64 form sums doubles (ds_add_f64); the FP32 form sums
 // 64-bit fixed-point integers (on gfx950 integer LDS atomics are an order
-// of magnitude faster than fp32 ones) and flushes fp32 values.
+// of magnitude faster than fp32 ones) and flushes fp32 values.  Inside
+// pic2d_step the FP32 tile sort copies no particles: the species' pass of
+// the sorting step reads them through the permutation and writes the other
+// buffer.
 #include <hip/hip_runtime.h>
 #include <new>
 #include <math.h>
