@@ -388,6 +388,44 @@ def test_sort_applied_by_the_pass_matches_the_copying_sort():
         assert np.array_equal(pa[sp], pb[sp]), sp
 
 
+def test_sort_applied_by_the_pass_at_the_128cubed_workload():
+    """The 128^3 x 64 ppc workload (268,435,456 particles) in the
+    deterministic deposit mode: 21 steps of pic_step, whose step 21 runs the
+    inflow and then the tile sort that the particle pass applies, against
+    20 steps, the C-ABI sort of both species (which copies the particles)
+    and a 21st step with the inflow only.  Every field plane bit-identical,
+    the counts equal, and the FP64 sums over the particles (which sit in
+    other slots) equal to their summation rounding (relative 1e-12)."""
+    out = []
+    for in_step in (True, False):
+        c = ps.preset(2)
+        c.deposit = ps.PIC_DEPOSIT_DETERMINISTIC
+        if not in_step:
+            c.sort_every = 1000
+        s = ps.Simulation(c)
+        s.load_particles()
+        s.begin()
+        if in_step:
+            s.step(21)
+        else:
+            s.step(20)
+            s.sort(0)
+            s.sort(1)
+            s.step(1)
+        out.append(({n: s.download_field(n) for n in FIELDS + J}, s.diagnose(),
+                    s.sort_stats()))
+        s.close()
+    (fa, da, sa), (fb, db, sb) = out
+    assert sa == sb and sa[0] == 2, (sa, sb)
+    for n in FIELDS + J:
+        assert np.array_equal(fa[n], fb[n]), n
+    assert np.array_equal(da["count"], db["count"])
+    for k in ("kinetic", "kinetic_nr"):
+        assert np.allclose(da[k], db[k], rtol=1e-12, atol=0), (k, da[k], db[k])
+    scale = np.abs(da["momentum"]).max()
+    assert np.allclose(da["momentum"], db["momentum"], rtol=0, atol=1e-12 * scale)
+
+
 def test_multi_step_runs_with_sort_and_inflow():
     c = cfg(1)
     s = ps.Simulation(c)
